@@ -1,0 +1,126 @@
+"""Benchmark: particle-steps/s of the MPM step on the 8M-particle slab.
+
+One step = bin/sort + P2G + grid update + fused G2P over all particles of the
+slab (4 x 0.5 x 4 m, 8,000,000 particles, 8 ppc, dx = 0.02, per-particle E,
+box collider, dt = 5e-5), inputs resident in HBM before the timed region.
+The roofline entry is the dominant kernel's algorithmic bytes per launch
+(84.1 B/particle for P2G, 131.5 B/particle for G2P, FP32 J_P layout) over its
+average launch time measured with HIP events on the engine's stream.
+The CPU baseline is the FP64 oracle (the reference algorithm restated, same
+serial/OpenMP split) timed on a bounded sample of the same slab.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "stomakhin-style-snow-simulator_amd"))
+
+HBM_PEAK_GBS = 8000.0
+ALG_BYTES = {"p2g": 84.1, "g2p": 131.5, "grid": 11.5}
+
+
+def cpu_baseline(seed):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import snowmpm
+    import mpmref  # the oracle driver, used here only as the timed CPU baseline
+    import oracle as orc
+
+    scale = 0.2  # 80 x 50 x 80 = 320,000 particles of the same slab
+    sc = snowmpm.generate_scene(2, seed=seed, scale=scale)
+    n = sc["x"].shape[0]
+    threads = os.cpu_count() or 1
+    threads = min(threads, 16)
+    prm = sc["params"]
+    mu0, la0 = snowmpm.lame(sc["E"], prm.nu)
+    st = orc.State(sc["x"], sc["v"], sc["m"], mu0=mu0, lambda0=la0)
+    st.setup(prm.dx)
+    cs = mpmref.orc_colliders(sc["colliders"])
+    op = mpmref.orc_params(prm)
+    st.run(cs, op, 1, nthreads=threads)  # warm-up step
+    steps = 3
+    t0 = time.perf_counter()
+    st.run(cs, op, steps, nthreads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"slab scale {scale}: {n} particles x {steps} steps (after 1 warm-up), FP64 oracle "
+                      f"(std::unordered_map grid, reference serial/OpenMP split), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
+              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import snowmpm
+
+    t0 = time.perf_counter()
+    sc = snowmpm.generate_scene(2, seed=args.seed, scale=1.0)
+    n = sc["x"].shape[0]
+    eng = snowmpm.Engine(capacity=n + 1024)
+    sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"], E=sc["E"])
+    eng.setup()
+    print(f"# slab: {n} particles resident, setup {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    if args.warmup:
+        eng.step(args.warmup)
+    timings = snowmpm.StepTimings()
+    t0 = time.perf_counter()
+    eng.step(args.steps, timings)  # synchronizes the device before returning
+    wall = time.perf_counter() - t0
+    kt = eng.kernel_times()
+    st, fb = eng.status(sid)
+    eng.close()
+
+    ms_step = wall * 1e3 / args.steps
+    value = n * args.steps / wall
+    per_kernel = {k: kt[k] for k in ("p2g", "g2p", "grid")}
+    dom = max(("p2g", "g2p"), key=lambda k: per_kernel[k])
+    achieved = ALG_BYTES[dom] * n / (per_kernel[dom] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get(dom)
+    out = {
+        "metric": "particle-steps/s (8M-particle slab)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "slab 4x0.5x4 m, 8,000,000 particles (8 ppc, dx=0.02, jitter 0.1), "
+                               "per-particle E~U[1e5,2e5], box collider, dt=5e-5, FP32 with FP64 SVD fallback",
+                   "particles": n, "layout": "FP32 SoA, J_P (no full F_P)"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_particle": ALG_BYTES[dom]},
+        "kernel_ms": {k: round(v, 4) for k, v in kt.items() if k != "steps"},
+        "status": st,
+        "fp64_fallbacks": fb,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.seed)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

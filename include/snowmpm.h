@@ -1,0 +1,163 @@
+/*
+ * snowmpm.h -- C ABI of the MI355X MPM step engine (libsnowmpm.so).
+ *
+ * Drop-in boundary for the reference solver loop.  Every entry point returns
+ * an int status (MPM_OK = 0) unless stated; plain pointers and sizes only.
+ * Host arrays are double precision in the reference's per-particle layout
+ * (Eigen::Vector3d / row-major Matrix3d per particle, contiguous).
+ *
+ * Reference interfaces replaced (paths relative to the reference repo):
+ *   setup()        include/snow/grid.hpp:96,  src/grid.cpp:340-357
+ *   step()         include/snow/grid.hpp:97,  src/grid.cpp:359-450
+ *   make_plane()   include/snow/collision.hpp:14, src/collision.cpp:5-27
+ *   StepTimings    include/snow/debug.hpp:3-15
+ *   Particle       include/snow/particle.hpp:5-28 (state layout of the arrays)
+ *   import_*()     include/snow/import.hpp:22-25, src/import.cpp:11-137
+ *   export_particles() include/snow/export.hpp:8, src/export.cpp:10-22
+ */
+#ifndef SNOWMPM_H
+#define SNOWMPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* return codes */
+#define MPM_OK 0
+#define MPM_ERR_ARG (-1)      /* bad argument (null pointer, bad id, bad size) */
+#define MPM_ERR_HIP (-2)      /* HIP runtime error */
+#define MPM_ERR_OOM (-3)      /* device allocation failed */
+#define MPM_ERR_CAPACITY (-4) /* particle capacity or scene table exhausted */
+#define MPM_ERR_STATE (-5)    /* call not valid in the engine's state */
+#define MPM_ERR_DEVICE (-6)   /* a kernel raised a status bit (see mpm_scene_status) */
+#define MPM_ERR_IO (-7)       /* file could not be read/written or was malformed */
+
+/* device status bits, per scene */
+#define MPM_STATUS_NAN 0x1u       /* non-finite particle state */
+#define MPM_STATUS_OVERFLOW 0x2u  /* fixed-point grid accumulator overflow */
+#define MPM_STATUS_HASH_FULL 0x4u /* block hash table full */
+#define MPM_STATUS_RANGE 0x8u     /* particle outside the block-coordinate range */
+
+typedef struct mpm_engine mpm_engine;
+
+/* The scalar arguments of step() (grid.hpp:97), plus nu for per-particle E. */
+typedef struct {
+    double gravity[3];
+    double dt;              /* time_step */
+    double dx;              /* grid_spacing */
+    double mu0;             /* mu_0 */
+    double lambda0;         /* lambda_0 */
+    double hardening;       /* hardening_coefficient (xi) */
+    double max_compression; /* theta_c */
+    double max_stretch;     /* theta_s */
+    double alpha;           /* FLIP ratio */
+    double nu;              /* Poisson ratio used with per-particle Young's moduli */
+} mpm_params;
+
+/* POD collider (replaces CollisionBody, collision.hpp:6-12).  type 0 is the
+ * reference's plane (make_plane, collision.cpp:5-27); 1 a solid axis-aligned
+ * box; 2 a solid sphere.  velocity: the body's constant velocity. */
+#define MPM_COLLIDER_PLANE 0
+#define MPM_COLLIDER_BOX 1
+#define MPM_COLLIDER_SPHERE 2
+typedef struct {
+    int type;
+    int pad_;
+    double point[3];
+    double normal[3];
+    double half_extent[3];
+    double radius;
+    double velocity[3];
+    double friction;
+} mpm_collider;
+
+/* StepTimings (debug.hpp:3-15), milliseconds, accumulated with +=.  Fused
+ * kernels report under the first reference stage they absorb:
+ *   stencil       <- block binning, sort and hash clear
+ *   p2g           <- P2G + force_constitutive + force_grid_accumulation
+ *   grid_velocity <- grid_velocity + grid_collisions
+ *   g2p           <- stress_update + G2P + particle_collisions + position_update
+ * and the absorbed stages read 0. */
+typedef struct {
+    double stencil, p2g, forces_constitutive, forces_accumulative, grid_velocity, grid_collisions, stress_update,
+        g2p, particle_collisions, position_update;
+} mpm_timings;
+
+typedef struct {
+    size_t particle_capacity; /* total particles of all resident scenes */
+    int hash_log2;            /* block hash slots = 2^hash_log2; 0 = sized from capacity */
+    int track_fp;             /* 1: keep the full plastic gradient F_P; 0: keep J_P = det F_P */
+    int device;               /* HIP device ordinal */
+    int pad_;
+    double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
+} mpm_config;
+
+/* ---- engine lifetime ---- */
+int mpm_create(const mpm_config* cfg, mpm_engine** out);
+int mpm_destroy(mpm_engine* e);
+const char* mpm_last_error(void);
+
+/* ---- scenes (a scene = one reference std::vector<Particle> + colliders) ----
+ * x, v: n*3; m: n; V0: n or NULL (then mpm_setup computes it, as setup());
+ * Fe, Fp: n*9 row-major or NULL (identity, particle.hpp:24-25);
+ * E: per-particle Young's modulus or NULL (use prm->mu0/lambda0). */
+int mpm_scene_add(mpm_engine* e, const mpm_params* prm, const mpm_collider* colliders, int ncolliders, size_t n,
+                  const double* x, const double* v, const double* m, const double* V0, const double* Fe,
+                  const double* Fp, const double* E, int* scene_id);
+int mpm_scene_remove(mpm_engine* e, int scene);
+int mpm_scene_count(mpm_engine* e, int scene, size_t* n);
+
+/* setup(): V_p0 for every scene added without V0 (grid.cpp:340-357). */
+int mpm_setup(mpm_engine* e);
+
+/* nsteps calls of step() on every active scene (grid.cpp:359-450).
+ * timings may be NULL; when given, per-stage times are added to it. */
+int mpm_step(mpm_engine* e, int nsteps, mpm_timings* timings);
+
+/* Copy a scene's state back in its original particle order.  Any output may be
+ * NULL.  Fp is only available with track_fp; Jp = det F_P always is. */
+int mpm_scene_download(mpm_engine* e, int scene, double* x, double* v, double* Fe, double* Fp, double* Jp,
+                       double* V0);
+
+/* Status bits OR-ed since the scene was added, and FP64 SVD fallback count. */
+int mpm_scene_status(mpm_engine* e, int scene, uint32_t* status, uint64_t* fp64_fallbacks);
+
+/* FP64 sums on the device: [sum m, sum m x (3), sum m v (3)]. */
+int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]);
+
+/* Average per-kernel milliseconds of the last timed mpm_step call:
+ * [bin+sort, p2g, grid, g2p(+fp64 fix-up), end]; *nsteps = steps averaged. */
+int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
+
+/* ---- collider factories (collision.cpp:5-27 and the box/sphere bodies) ---- */
+mpm_collider mpm_make_plane(const double point[3], const double normal[3], double friction);
+mpm_collider mpm_make_box(const double centre[3], const double half_extent[3], double friction);
+mpm_collider mpm_make_sphere(const double centre[3], double radius, double friction);
+
+/* ---- host I/O in the reference formats ----
+ * groups.csv: group_id,vx,vy,vz,mass; particles.csv: group_id,x,y,z
+ * (import.cpp:37-137).  On success *n is the particle count and x, v, m are
+ * malloc'ed (release with mpm_free).  Errors return MPM_ERR_IO and the
+ * reference's message is in mpm_last_error(). */
+int mpm_import_particles(const char* groups_csv, const char* particles_csv, size_t* n, double** x, double** v,
+                         double** m);
+/* output/frame_%06d.csv writer (export.cpp:10-22): header x,y,z, rows x,-z,y. */
+int mpm_export_frame(const char* directory, int frame_no, size_t n, const double* x);
+void mpm_free(void* p);
+
+/* ---- synthetic workloads (seeded, identical for oracle and GPU) ----
+ * kind 0: snow cube (64,000 particles); 1: two tori; 2: slab; 3: one scene of
+ * the batched distribution (index selects it).  scale < 1 shrinks the shape
+ * (same spacing) for parity-sized runs; 1 = the full configuration.
+ * Outputs are malloc'ed: x, v (n*3), m, E (n; E may be NULL if unused).
+ * prm / colliders receive the configuration's parameters. */
+int mpm_scene_generate(int kind, uint64_t seed, int index, double scale, size_t* n, double** x, double** v,
+                       double** m, double** E, mpm_params* prm, mpm_collider* colliders, int* ncolliders);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+// mpm_device.hpp -- device-side data layout of the MI355X MPM step engine.
+//
+// Particles live in HBM as FP32 structure-of-arrays, double buffered: the
+// G2P kernel reads buffer `cur` through the block-sorted index and writes
+// buffer `cur^1` in sorted order, so every step leaves particles grouped by
+// their 4^3-node home block.  The grid is block sparse: a device hash maps a
+// packed (scene, bx, by, bz) key to a slot, and the slot index is the block's
+// storage index in every per-block array (no indirection, no compaction).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace snowmpm {
+
+constexpr int kMaxColliders = 8;
+constexpr int kMaxScenes = 8191;            // scene id 8191 is reserved (empty key)
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int kCoordBits = 17;              // per axis, biased
+constexpr int kCoordBias = 1 << 16;         // block coordinate range +-65534
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr int kBlockNodes = 64;             // 4x4x4 nodes per block
+constexpr int kGridCh = 7;                  // int64 accumulators: m, p(3), f(3)
+constexpr int kVelCh = 6;                   // FP32 outputs: v(3), u(3) (u = PIC/FLIP blend)
+constexpr int kStressCh = 6;                // symmetric force matrix -V0 P F^T
+
+// Status bits (per scene, OR-ed by kernels).
+constexpr uint32_t kStatusNaN = 0x1;
+constexpr uint32_t kStatusOverflow = 0x2;
+constexpr uint32_t kStatusHashFull = 0x4;
+constexpr uint32_t kStatusRange = 0x8;
+
+struct ColliderDev {
+    int type;  // 0 plane, 1 box, 2 sphere
+    float point[3];
+    float normal[3];
+    float half_extent[3];
+    float radius;
+    float velocity[3];
+    float friction;
+};
+
+struct SceneDev {
+    float dx, inv_dx, dt;
+    float gravity[3];
+    float mu0, lambda0, nu;
+    float hardening, theta_c, theta_s, alpha;
+    float scale_m, scale_p, scale_f;         // float -> fixed point (powers of two)
+    double inv_scale_m, inv_scale_p, inv_scale_f;
+    int use_E;                               // per-particle Young's modulus present
+    int ncoll;
+    int flags;                               // kSceneActive | kSceneSetup
+    int pad_;
+    ColliderDev coll[kMaxColliders];
+};
+constexpr int kSceneActive = 1;
+constexpr int kSceneSetup = 2;
+
+// One particle buffer (SoA, FP32).  Fp is null unless the engine tracks the
+// full plastic deformation gradient; otherwise J_P = det(F_P) carries it.
+struct PBuf {
+    float* x[3];
+    float* v[3];
+    float* F[9];   // elastic deformation gradient F_E, row major
+    float* Jp;     // det F_P
+    float* Fp[9];  // optional full F_P
+    float* m;
+    float* V0;
+    float* E;      // per-particle Young's modulus (used if scene.use_E)
+    float* S[kStressCh];  // force matrix for the next P2G: S00 S11 S22 S01 S02 S12
+    uint32_t* pid; // particle index within its scene (download order, sort tie-break)
+    uint32_t* sid; // scene id
+};
+
+struct Counters {
+    uint32_t n_particles;  // particles in the current buffer
+    uint32_t n_home;       // blocks holding particles
+    uint32_t n_all;        // home + neighbour blocks
+    uint32_t cursor;       // particles binned this pass
+    uint32_t n_fallback;   // particles deferred to the FP64 fix-up this pass
+    uint32_t global_status;
+    uint32_t pad_[2];
+};
+
+struct GridDev {
+    uint64_t* keys;        // [H] hash keys, kEmptyKey when free
+    uint32_t mask;         // H - 1
+    uint32_t max_probe;
+    uint32_t* cellcnt;     // [H*64] particles per home cell (zero between passes)
+    uint32_t* celloff;     // [H*64] exclusive offsets within the block
+    uint32_t* blkoff;      // [H] first sorted position of the block
+    uint32_t* blkcnt;      // [H] particles in the block
+    uint32_t* nb;          // [H*8] slots of the block + {0,1}^3 neighbours
+    unsigned long long* acc;  // [H*7*64] fixed-point accumulators (zero between passes)
+    float* vu;             // [H*6*64] grid velocity and PIC/FLIP blend
+    uint32_t* home_list;   // [H]
+    uint32_t* all_list;    // [H]
+};
+
+struct BinDev {
+    uint32_t* pkey;        // [N] slot<<6 | cell, or kInvalid
+    uint32_t* prank;       // [N] rank within cell
+    uint32_t* sidx;        // [N] sorted position -> particle index
+    uint32_t* spid;        // [N] sorted position -> pid (sort key inside a cell)
+};
+
+struct FallbackDev {
+    uint32_t* pos;         // [N] sorted positions whose update is redone in FP64
+    float* L;              // [N*9] their velocity gradients
+};
+
+struct StatusDev {
+    uint32_t* scene_status;          // [kMaxScenes]
+    unsigned long long* fp64_count;  // [kMaxScenes] FP64 SVD fallbacks
+    float* vmax;                     // [kMaxScenes] scratch for scale reductions
+};
+
+__host__ __device__ inline uint64_t pack_key(uint32_t sid, int bx, int by, int bz) {
+    return ((uint64_t)sid << 51) | ((uint64_t)(uint32_t)(bx + kCoordBias) << 34) |
+           ((uint64_t)(uint32_t)(by + kCoordBias) << 17) | (uint64_t)(uint32_t)(bz + kCoordBias);
+}
+__host__ __device__ inline void unpack_key(uint64_t k, uint32_t& sid, int& bx, int& by, int& bz) {
+    const uint64_t m = (1ull << kCoordBits) - 1;
+    sid = (uint32_t)(k >> 51);
+    bx = (int)((k >> 34) & m) - kCoordBias;
+    by = (int)((k >> 17) & m) - kCoordBias;
+    bz = (int)(k & m) - kCoordBias;
+}
+
+}  // namespace snowmpm

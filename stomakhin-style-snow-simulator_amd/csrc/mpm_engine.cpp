@@ -1,0 +1,623 @@
+// mpm_engine.cpp -- C++20 host engine behind the C ABI (include/snowmpm.h).
+//
+// Owns the device state of many resident scenes (particle double buffer,
+// block hash grid, binning arrays, status words) and drives the kernels of
+// mpm_kernels.hip on one HIP stream.  All scenes step together: one launch
+// per stage covers every active scene, because block keys carry the scene id.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/snowmpm.h"
+#include "mpm_device.hpp"
+#include "mpm_engine.hpp"
+#include "mpm_kernels.hpp"
+
+namespace snowmpm {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string& s) { g_last_error = s; }
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+            return MPM_ERR_HIP;                                                                \
+        }                                                                                      \
+    } while (0)
+
+static double pow2_scale(double ref) {
+    // float -> fixed point: ref maps to 2^40 (headroom 2^22 before int64 overflow)
+    if (!(ref > 0.0) || !std::isfinite(ref)) ref = 1.0;
+    int e = (int)std::ceil(std::log2(ref));
+    return std::ldexp(1.0, 40 - e);
+}
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+    for (void* p : allocs_) (void)hipFree(p);
+    allocs_.clear();
+    for (auto& ev : events_) (void)hipEventDestroy(ev);
+    events_.clear();
+    if (stream_) (void)hipStreamDestroy(stream_);
+    stream_ = nullptr;
+    if (host_stage_) (void)hipHostFree(host_stage_);
+    host_stage_ = nullptr;
+}
+
+int Engine::dalloc(void** p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+        return MPM_ERR_OOM;
+    }
+    allocs_.push_back(*p);
+    return MPM_OK;
+}
+
+int Engine::init(const mpm_config& cfg) {
+    cfg_ = cfg;
+    if (cfg_.particle_capacity == 0 || cfg_.particle_capacity > 0x7fffffffull) {
+        set_error("particle_capacity must be in [1, 2^31)");
+        return MPM_ERR_ARG;
+    }
+    if (cfg_.kappa <= 0) cfg_.kappa = 1.5;
+    HIP_TRY(hipSetDevice(cfg_.device));
+    HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    int cus = 256;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg_.device));
+    const size_t N = cfg_.particle_capacity;
+    launch_.grid_p = (int)std::min<size_t>((N + 255) / 256, (size_t)cus * 16);
+    launch_.grid_b = cus * 8;
+    launch_.grid_w = cus * 8;
+
+    int hl = cfg_.hash_log2;
+    if (hl <= 0) {
+        size_t want = std::max<size_t>(65536, N / 16);
+        hl = 16;
+        while (((size_t)1 << hl) < want) ++hl;
+    }
+    if (hl > 26) {
+        set_error("hash_log2 must be <= 26");
+        return MPM_ERR_ARG;
+    }
+    H_ = 1u << hl;
+    const bool track = cfg_.track_fp != 0;
+
+    int rc;
+    for (int b = 0; b < 2; ++b) {
+        PBuf& p = buf_[b];
+        std::memset(&p, 0, sizeof p);
+        const int nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1 + 6;
+        float* base;
+        if ((rc = dalloc((void**)&base, N * sizeof(float) * nf))) return rc;
+        float* q = base;
+        auto take = [&]() { float* r = q; q += N; return r; };
+        for (int e = 0; e < 3; ++e) p.x[e] = take();
+        for (int e = 0; e < 3; ++e) p.v[e] = take();
+        for (int e = 0; e < 9; ++e) p.F[e] = take();
+        p.Jp = take();
+        if (track)
+            for (int e = 0; e < 9; ++e) p.Fp[e] = take();
+        p.m = take();
+        p.V0 = take();
+        p.E = take();
+        for (int e = 0; e < 6; ++e) p.S[e] = take();
+        if ((rc = dalloc((void**)&p.pid, N * 4))) return rc;
+        if ((rc = dalloc((void**)&p.sid, N * 4))) return rc;
+    }
+    if ((rc = dalloc((void**)&bin_.pkey, N * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.prank, N * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.sidx, N * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.spid, N * 4))) return rc;
+    if ((rc = dalloc((void**)&fb_.pos, N * 4))) return rc;
+    if ((rc = dalloc((void**)&fb_.L, N * 36))) return rc;
+
+    grid_.mask = H_ - 1;
+    grid_.max_probe = 4096;
+    if ((rc = dalloc((void**)&grid_.keys, (size_t)H_ * 8))) return rc;
+    if ((rc = dalloc((void**)&grid_.cellcnt, (size_t)H_ * 64 * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.celloff, (size_t)H_ * 64 * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.blkoff, (size_t)H_ * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.blkcnt, (size_t)H_ * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.nb, (size_t)H_ * 8 * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.acc, (size_t)H_ * kGridCh * 64 * 8))) return rc;
+    if ((rc = dalloc((void**)&grid_.vu, (size_t)H_ * kVelCh * 64 * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.home_list, (size_t)H_ * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.all_list, (size_t)H_ * 4))) return rc;
+    HIP_TRY(hipMemsetAsync(grid_.keys, 0xFF, (size_t)H_ * 8, stream_));
+    HIP_TRY(hipMemsetAsync(grid_.cellcnt, 0, (size_t)H_ * 64 * 4, stream_));
+    HIP_TRY(hipMemsetAsync(grid_.acc, 0, (size_t)H_ * kGridCh * 64 * 8, stream_));
+
+    if ((rc = dalloc((void**)&cnt_, sizeof(Counters)))) return rc;
+    HIP_TRY(hipMemsetAsync(cnt_, 0, sizeof(Counters), stream_));
+    if ((rc = dalloc((void**)&d_scenes_, sizeof(SceneDev) * kMaxScenes))) return rc;
+    HIP_TRY(hipMemsetAsync(d_scenes_, 0, sizeof(SceneDev) * kMaxScenes, stream_));
+    if ((rc = dalloc((void**)&st_.scene_status, 4 * kMaxScenes))) return rc;
+    if ((rc = dalloc((void**)&st_.fp64_count, 8 * kMaxScenes))) return rc;
+    if ((rc = dalloc((void**)&st_.vmax, 4 * kMaxScenes))) return rc;
+    HIP_TRY(hipMemsetAsync(st_.scene_status, 0, 4 * kMaxScenes, stream_));
+    HIP_TRY(hipMemsetAsync(st_.fp64_count, 0, 8 * kMaxScenes, stream_));
+    if ((rc = dalloc((void**)&d_sums_, sizeof(double) * 7 * 1024))) return rc;
+    HIP_TRY(hipStreamSynchronize(stream_));
+    h_scenes_.assign(kMaxScenes, SceneDev{});
+    scenes_.clear();
+    return MPM_OK;
+}
+
+StepArgs Engine::args(uint32_t pass_flag) const {
+    StepArgs a;
+    std::memset(&a, 0, sizeof a);
+    a.cur = buf_[cur_];
+    a.nxt = buf_[cur_ ^ 1];
+    a.g = grid_;
+    a.bin = bin_;
+    a.st = st_;
+    a.fb = fb_;
+    a.cnt = cnt_;
+    a.scenes = d_scenes_;
+    a.pass_flag = pass_flag;
+    a.mass_only = 0;
+    a.copy_E = any_E_ ? 1 : 0;
+    a.kappa = (float)cfg_.kappa;
+    return a;
+}
+
+int Engine::device_count(uint32_t* n) {
+    Counters c;
+    HIP_TRY(hipMemcpyAsync(&c, cnt_, sizeof c, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    *n = c.n_particles;
+    return MPM_OK;
+}
+
+int Engine::upload_scene_table(int sid) {
+    HIP_TRY(hipMemcpyAsync(d_scenes_ + sid, &h_scenes_[sid], sizeof(SceneDev), hipMemcpyHostToDevice, stream_));
+    return MPM_OK;
+}
+
+int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll, size_t n, const double* x,
+                      const double* v, const double* m, const double* V0, const double* Fe, const double* Fp,
+                      const double* E, int* out_sid) {
+    if (!x || !v || !m || n == 0 || ncoll < 0 || ncoll > kMaxColliders || (ncoll > 0 && !coll)) {
+        set_error("mpm_scene_add: bad arguments");
+        return MPM_ERR_ARG;
+    }
+    if (!(prm.dx > 0) || !(prm.dt > 0)) {
+        set_error("mpm_scene_add: dx and dt must be positive");
+        return MPM_ERR_ARG;
+    }
+    int sid = -1;
+    for (int s = 0; s < (int)scenes_.size(); ++s)
+        if (!scenes_[s].used && !scenes_[s].pending_drop) {
+            sid = s;
+            break;
+        }
+    if (sid < 0) {
+        if ((int)scenes_.size() >= kMaxScenes - 1) {
+            set_error("scene table full");
+            return MPM_ERR_CAPACITY;
+        }
+        sid = (int)scenes_.size();
+        scenes_.push_back(SceneHost{});
+    }
+    uint32_t n0;
+    int rc = device_count(&n0);
+    if (rc) return rc;
+    if ((size_t)n0 + n > cfg_.particle_capacity) {
+        set_error("particle capacity exceeded: " + std::to_string(n0) + " + " + std::to_string(n) + " > " +
+                  std::to_string(cfg_.particle_capacity));
+        return MPM_ERR_CAPACITY;
+    }
+    const bool track = cfg_.track_fp != 0;
+    // Host staging: SoA float fields in device-buffer order.
+    const int nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1;
+    std::vector<float> stage((size_t)nf * n);
+    std::vector<uint32_t> pid(n), sidv(n, (uint32_t)sid);
+    double mmax = 0, vmax2 = 0, fref = 0;
+    const double nu = prm.nu;
+    for (size_t i = 0; i < n; ++i) {
+        size_t o = 0;
+        for (int e = 0; e < 3; ++e, o += n) stage[o + i] = (float)x[i * 3 + e];
+        for (int e = 0; e < 3; ++e, o += n) stage[o + i] = (float)v[i * 3 + e];
+        double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if (Fe) std::memcpy(F, Fe + i * 9, sizeof F);
+        if (Fp) std::memcpy(P, Fp + i * 9, sizeof P);
+        for (int e = 0; e < 9; ++e, o += n) stage[o + i] = (float)F[e];
+        const double jp = P[0] * (P[4] * P[8] - P[5] * P[7]) - P[1] * (P[3] * P[8] - P[5] * P[6]) +
+                          P[2] * (P[3] * P[7] - P[4] * P[6]);
+        stage[o + i] = (float)jp;
+        o += n;
+        if (track)
+            for (int e = 0; e < 9; ++e, o += n) stage[o + i] = (float)P[e];
+        stage[o + i] = (float)m[i];
+        o += n;
+        stage[o + i] = V0 ? (float)V0[i] : 0.0f;
+        o += n;
+        stage[o + i] = E ? (float)E[i] : 0.0f;
+        pid[i] = (uint32_t)i;
+        mmax = std::max(mmax, std::fabs(m[i]));
+        vmax2 = std::max(vmax2, v[i * 3] * v[i * 3] + v[i * 3 + 1] * v[i * 3 + 1] + v[i * 3 + 2] * v[i * 3 + 2]);
+        if (V0) {
+            double mu = prm.mu0, la = prm.lambda0;
+            if (E) {
+                mu = E[i] / (2 * (1 + nu));
+                la = E[i] * nu / ((1 + nu) * (1 - 2 * nu));
+            }
+            fref = std::max(fref, std::fabs(V0[i]) * (mu + la));
+        }
+    }
+    const PBuf& p = buf_[cur_];
+    {
+        size_t o = 0;
+        auto put = [&](float* dst) -> hipError_t {
+            hipError_t e = hipMemcpyAsync(dst + n0, stage.data() + o, n * 4, hipMemcpyHostToDevice, stream_);
+            o += n;
+            return e;
+        };
+        for (int e = 0; e < 3; ++e) HIP_TRY(put(p.x[e]));
+        for (int e = 0; e < 3; ++e) HIP_TRY(put(p.v[e]));
+        for (int e = 0; e < 9; ++e) HIP_TRY(put(p.F[e]));
+        HIP_TRY(put(p.Jp));
+        if (track)
+            for (int e = 0; e < 9; ++e) HIP_TRY(put(p.Fp[e]));
+        HIP_TRY(put(p.m));
+        HIP_TRY(put(p.V0));
+        HIP_TRY(put(p.E));
+        HIP_TRY(hipMemcpyAsync(p.pid + n0, pid.data(), n * 4, hipMemcpyHostToDevice, stream_));
+        HIP_TRY(hipMemcpyAsync(p.sid + n0, sidv.data(), n * 4, hipMemcpyHostToDevice, stream_));
+    }
+    SceneDev& sd = h_scenes_[sid];
+    std::memset(&sd, 0, sizeof sd);
+    sd.dx = (float)prm.dx;
+    sd.inv_dx = (float)(1.0 / prm.dx);
+    sd.dt = (float)prm.dt;
+    for (int e = 0; e < 3; ++e) sd.gravity[e] = (float)prm.gravity[e];
+    sd.mu0 = (float)prm.mu0;
+    sd.lambda0 = (float)prm.lambda0;
+    sd.nu = (float)prm.nu;
+    sd.hardening = (float)prm.hardening;
+    sd.theta_c = (float)prm.max_compression;
+    sd.theta_s = (float)prm.max_stretch;
+    sd.alpha = (float)prm.alpha;
+    const double sm = pow2_scale(mmax), sp = pow2_scale(mmax * std::max(1.0, std::sqrt(vmax2)));
+    const double sf = pow2_scale(V0 ? fref / prm.dx : 1.0);
+    sd.scale_m = (float)sm;
+    sd.scale_p = (float)sp;
+    sd.scale_f = (float)sf;
+    sd.inv_scale_m = 1.0 / sm;
+    sd.inv_scale_p = 1.0 / sp;
+    sd.inv_scale_f = 1.0 / sf;
+    sd.use_E = E ? 1 : 0;
+    sd.ncoll = ncoll;
+    for (int k = 0; k < ncoll; ++k) {
+        ColliderDev& c = sd.coll[k];
+        c.type = coll[k].type;
+        for (int e = 0; e < 3; ++e) {
+            c.point[e] = (float)coll[k].point[e];
+            c.normal[e] = (float)coll[k].normal[e];
+            c.half_extent[e] = (float)coll[k].half_extent[e];
+            c.velocity[e] = (float)coll[k].velocity[e];
+        }
+        c.radius = (float)coll[k].radius;
+        c.friction = (float)coll[k].friction;
+    }
+    sd.flags = V0 ? kSceneActive : kSceneSetup;
+    if (E) any_E_ = true;
+    if ((rc = upload_scene_table(sid))) return rc;
+    HIP_TRY(hipMemsetAsync(st_.scene_status + sid, 0, 4, stream_));
+    HIP_TRY(hipMemsetAsync(st_.fp64_count + sid, 0, 8, stream_));
+    launch_set_count(cnt_, n0 + (uint32_t)n, stream_);
+    StepArgs a = args(kSceneActive);
+    launch_init_stress(a, n0, (uint32_t)n, stream_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    SceneHost& sh = scenes_[sid];
+    sh = SceneHost{};
+    sh.used = true;
+    sh.n = n;
+    sh.prm = prm;
+    *out_sid = sid;
+    return MPM_OK;
+}
+
+int Engine::scene_remove(int sid) {
+    if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) {
+        set_error("bad scene id");
+        return MPM_ERR_ARG;
+    }
+    // Particles of a scene with no flags are dropped by the next binning pass
+    // (G2P only writes binned particles), which compacts the buffer.
+    h_scenes_[sid].flags = 0;
+    int rc = upload_scene_table(sid);
+    if (rc) return rc;
+    scenes_[sid].used = false;
+    scenes_[sid].pending_drop = true;
+    return MPM_OK;
+}
+
+int Engine::setup() {
+    bool any = false;
+    for (size_t s = 0; s < scenes_.size(); ++s)
+        if (scenes_[s].used && (h_scenes_[s].flags & kSceneSetup)) any = true;
+    if (!any) return MPM_OK;
+    StepArgs a = args(kSceneSetup);
+    launch_setup(a, launch_, stream_);
+    HIP_TRY(hipGetLastError());
+    // Force scale from the new volumes: max V0 (mu0 + lambda0) per scene.
+    HIP_TRY(hipMemsetAsync(st_.vmax, 0, 4 * kMaxScenes, stream_));
+    launch_scene_max(a, launch_, 0, stream_);
+    std::vector<float> vmax(scenes_.size());
+    HIP_TRY(hipMemcpyAsync(vmax.data(), st_.vmax, 4 * scenes_.size(), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (size_t s = 0; s < scenes_.size(); ++s) {
+        if (!scenes_[s].used || !(h_scenes_[s].flags & kSceneSetup)) continue;
+        SceneDev& sd = h_scenes_[s];
+        const double sf = pow2_scale((double)vmax[s] / (double)sd.dx);
+        sd.scale_f = (float)sf;
+        sd.inv_scale_f = 1.0 / sf;
+        sd.flags = kSceneActive;
+        int rc = upload_scene_table((int)s);
+        if (rc) return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return check_status();
+}
+
+int Engine::check_status() {
+    if (scenes_.empty()) return MPM_OK;
+    std::vector<uint32_t> st(scenes_.size());
+    HIP_TRY(hipMemcpyAsync(st.data(), st_.scene_status, 4 * scenes_.size(), hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (size_t s = 0; s < scenes_.size(); ++s)
+        if (scenes_[s].used && st[s]) {
+            char buf[128];
+            std::snprintf(buf, sizeof buf, "scene %zu raised device status 0x%x", s, st[s]);
+            set_error(buf);
+            return MPM_ERR_DEVICE;
+        }
+    return MPM_OK;
+}
+
+int Engine::step(int nsteps, mpm_timings* t) {
+    if (nsteps < 0) {
+        set_error("nsteps < 0");
+        return MPM_ERR_ARG;
+    }
+    int rc = setup();
+    if (rc) return rc;
+    const bool timed = t != nullptr;
+    if (timed) {
+        const size_t need = (size_t)nsteps * kStepEvents;
+        while (events_.size() < need) {
+            hipEvent_t ev;
+            HIP_TRY(hipEventCreate(&ev));
+            events_.push_back(ev);
+        }
+    }
+    for (int s = 0; s < nsteps; ++s) {
+        StepArgs a = args(kSceneActive);
+        launch_step(a, launch_, stream_, timed ? &events_[(size_t)s * kStepEvents] : nullptr);
+        cur_ ^= 1;
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    if (nsteps > 0)
+        for (auto& sh : scenes_) sh.pending_drop = false;
+    if (timed) {
+        double k[5] = {0, 0, 0, 0, 0};
+        for (int s = 0; s < nsteps; ++s)
+            for (int j = 0; j < 5; ++j) {
+                float ms = 0;
+                HIP_TRY(hipEventElapsedTime(&ms, events_[(size_t)s * kStepEvents + j],
+                                            events_[(size_t)s * kStepEvents + j + 1]));
+                k[j] += ms;
+            }
+        t->stencil += k[0] + k[4];
+        t->p2g += k[1];
+        t->grid_velocity += k[2];
+        t->g2p += k[3];
+        for (int j = 0; j < 5; ++j) kernel_ms_[j] = nsteps ? k[j] / nsteps : 0.0;
+        kernel_steps_ = nsteps;
+    }
+    return check_status();
+}
+
+int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0) {
+    if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) {
+        set_error("bad scene id");
+        return MPM_ERR_ARG;
+    }
+    const bool track = cfg_.track_fp != 0;
+    if (Fp && !track) {
+        set_error("F_P download needs track_fp");
+        return MPM_ERR_STATE;
+    }
+    const size_t n = scenes_[sid].n;
+    const int nf = 17 + (track ? 9 : 0);
+    float* d_out = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_out, (size_t)nf * n * 4, stream_));
+    StepArgs a = args(kSceneActive);
+    launch_gather_scene(a, launch_, (uint32_t)sid, d_out, (uint32_t)n, track ? 1 : 0, stream_);
+    std::vector<float> h((size_t)nf * n);
+    HIP_TRY(hipMemcpyAsync(h.data(), d_out, h.size() * 4, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipFreeAsync(d_out, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n; ++i) {
+        size_t o = 0;
+        for (int e = 0; e < 3; ++e, o += n)
+            if (x) x[i * 3 + e] = h[o + i];
+        for (int e = 0; e < 3; ++e, o += n)
+            if (v) v[i * 3 + e] = h[o + i];
+        for (int e = 0; e < 9; ++e, o += n)
+            if (Fe) Fe[i * 9 + e] = h[o + i];
+        if (Jp) Jp[i] = h[o + i];
+        o += n;
+        if (V0) V0[i] = h[o + i];
+        o += n;
+        if (track)
+            for (int e = 0; e < 9; ++e, o += n)
+                if (Fp) Fp[i * 9 + e] = h[o + i];
+    }
+    return MPM_OK;
+}
+
+int Engine::scene_status(int sid, uint32_t* status, uint64_t* fb) {
+    if (sid < 0 || sid >= (int)scenes_.size()) {
+        set_error("bad scene id");
+        return MPM_ERR_ARG;
+    }
+    uint32_t s = 0;
+    unsigned long long c = 0;
+    HIP_TRY(hipMemcpyAsync(&s, st_.scene_status + sid, 4, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipMemcpyAsync(&c, st_.fp64_count + sid, 8, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    if (status) *status = s;
+    if (fb) *fb = c;
+    return MPM_OK;
+}
+
+int Engine::scene_sums(int sid, double out[7]) {
+    if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) {
+        set_error("bad scene id");
+        return MPM_ERR_ARG;
+    }
+    const int nb = 1024;
+    StepArgs a = args(kSceneActive);
+    launch_scene_sums(a, (uint32_t)sid, d_sums_, nb, stream_);
+    std::vector<double> h((size_t)nb * 7);
+    HIP_TRY(hipMemcpyAsync(h.data(), d_sums_, h.size() * 8, hipMemcpyDeviceToHost, stream_));
+    HIP_TRY(hipStreamSynchronize(stream_));
+    for (int e = 0; e < 7; ++e) {
+        double s = 0;
+        for (int b = 0; b < nb; ++b) s += h[(size_t)b * 7 + e];
+        out[e] = s;
+    }
+    return MPM_OK;
+}
+
+}  // namespace snowmpm
+
+// ---------------------------------------------------------------------------
+// C ABI
+using snowmpm::Engine;
+using snowmpm::set_error;
+
+struct mpm_engine {
+    Engine eng;
+};
+
+extern "C" {
+
+const char* mpm_last_error(void) { return snowmpm::g_last_error.c_str(); }
+
+int mpm_create(const mpm_config* cfg, mpm_engine** out) {
+    if (!cfg || !out) {
+        set_error("mpm_create: null argument");
+        return MPM_ERR_ARG;
+    }
+    *out = nullptr;
+    mpm_engine* e = new (std::nothrow) mpm_engine();
+    if (!e) return MPM_ERR_OOM;
+    int rc = e->eng.init(*cfg);
+    if (rc) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return MPM_OK;
+}
+
+int mpm_destroy(mpm_engine* e) {
+    if (!e) return MPM_ERR_ARG;
+    delete e;
+    return MPM_OK;
+}
+
+int mpm_scene_add(mpm_engine* e, const mpm_params* prm, const mpm_collider* colliders, int ncolliders, size_t n,
+                  const double* x, const double* v, const double* m, const double* V0, const double* Fe,
+                  const double* Fp, const double* E, int* scene_id) {
+    if (!e || !prm || !scene_id) {
+        set_error("mpm_scene_add: null argument");
+        return MPM_ERR_ARG;
+    }
+    return e->eng.scene_add(*prm, colliders, ncolliders, n, x, v, m, V0, Fe, Fp, E, scene_id);
+}
+
+int mpm_scene_remove(mpm_engine* e, int scene) { return e ? e->eng.scene_remove(scene) : MPM_ERR_ARG; }
+
+int mpm_scene_count(mpm_engine* e, int scene, size_t* n) {
+    if (!e || !n) return MPM_ERR_ARG;
+    return e->eng.scene_count(scene, n);
+}
+
+int mpm_setup(mpm_engine* e) { return e ? e->eng.setup() : MPM_ERR_ARG; }
+
+int mpm_step(mpm_engine* e, int nsteps, mpm_timings* t) { return e ? e->eng.step(nsteps, t) : MPM_ERR_ARG; }
+
+int mpm_scene_download(mpm_engine* e, int scene, double* x, double* v, double* Fe, double* Fp, double* Jp,
+                       double* V0) {
+    return e ? e->eng.scene_download(scene, x, v, Fe, Fp, Jp, V0) : MPM_ERR_ARG;
+}
+
+int mpm_scene_status(mpm_engine* e, int scene, uint32_t* status, uint64_t* fb) {
+    return e ? e->eng.scene_status(scene, status, fb) : MPM_ERR_ARG;
+}
+
+int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]) {
+    return (e && out7) ? e->eng.scene_sums(scene, out7) : MPM_ERR_ARG;
+}
+
+int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps) {
+    if (!e || !ms) return MPM_ERR_ARG;
+    e->eng.kernel_times(ms, nsteps);
+    return MPM_OK;
+}
+
+mpm_collider mpm_make_plane(const double point[3], const double normal[3], double friction) {
+    mpm_collider c;
+    std::memset(&c, 0, sizeof c);
+    c.type = MPM_COLLIDER_PLANE;
+    const double l = std::sqrt(normal[0] * normal[0] + normal[1] * normal[1] + normal[2] * normal[2]);
+    for (int a = 0; a < 3; ++a) {
+        c.point[a] = point[a];
+        c.normal[a] = normal[a] / l;  // normal.normalized(), collision.cpp:7
+    }
+    c.friction = friction;
+    return c;
+}
+
+mpm_collider mpm_make_box(const double centre[3], const double half_extent[3], double friction) {
+    mpm_collider c;
+    std::memset(&c, 0, sizeof c);
+    c.type = MPM_COLLIDER_BOX;
+    for (int a = 0; a < 3; ++a) {
+        c.point[a] = centre[a];
+        c.half_extent[a] = half_extent[a];
+    }
+    c.friction = friction;
+    return c;
+}
+
+mpm_collider mpm_make_sphere(const double centre[3], double radius, double friction) {
+    mpm_collider c;
+    std::memset(&c, 0, sizeof c);
+    c.type = MPM_COLLIDER_SPHERE;
+    for (int a = 0; a < 3; ++a) c.point[a] = centre[a];
+    c.radius = radius;
+    c.friction = friction;
+    return c;
+}
+
+void mpm_free(void* p) { std::free(p); }
+
+}  // extern "C"

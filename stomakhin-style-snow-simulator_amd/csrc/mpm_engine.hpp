@@ -1,0 +1,83 @@
+// mpm_engine.hpp -- host engine class behind the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/snowmpm.h"
+#include "mpm_device.hpp"
+#include "mpm_kernels.hpp"
+
+namespace snowmpm {
+
+extern thread_local std::string g_last_error;
+void set_error(const std::string& s);
+
+struct SceneHost {
+    bool used = false;
+    bool pending_drop = false;  // removed; particles leave at the next step
+    size_t n = 0;
+    mpm_params prm{};
+};
+
+class Engine {
+  public:
+    Engine() = default;
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    int init(const mpm_config& cfg);
+    int scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll, size_t n, const double* x,
+                  const double* v, const double* m, const double* V0, const double* Fe, const double* Fp,
+                  const double* E, int* sid);
+    int scene_remove(int sid);
+    int scene_count(int sid, size_t* n) const {
+        if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) return MPM_ERR_ARG;
+        *n = scenes_[sid].n;
+        return MPM_OK;
+    }
+    int setup();
+    int step(int nsteps, mpm_timings* t);
+    int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
+    int scene_status(int sid, uint32_t* status, uint64_t* fb);
+    int scene_sums(int sid, double out[7]);
+    void kernel_times(double ms[5], int* nsteps) const {
+        for (int j = 0; j < 5; ++j) ms[j] = kernel_ms_[j];
+        if (nsteps) *nsteps = kernel_steps_;
+    }
+
+  private:
+    int dalloc(void** p, size_t bytes);
+    void release();
+    StepArgs args(uint32_t pass_flag) const;
+    int device_count(uint32_t* n);
+    int upload_scene_table(int sid);
+    int check_status();
+
+    mpm_config cfg_{};
+    hipStream_t stream_ = nullptr;
+    LaunchCfg launch_{};
+    uint32_t H_ = 0;
+    PBuf buf_[2]{};
+    int cur_ = 0;
+    GridDev grid_{};
+    BinDev bin_{};
+    FallbackDev fb_{};
+    StatusDev st_{};
+    Counters* cnt_ = nullptr;
+    SceneDev* d_scenes_ = nullptr;
+    double* d_sums_ = nullptr;
+    void* host_stage_ = nullptr;
+    bool any_E_ = false;
+    std::vector<SceneDev> h_scenes_;
+    std::vector<SceneHost> scenes_;
+    std::vector<void*> allocs_;
+    std::vector<hipEvent_t> events_;
+    double kernel_ms_[5] = {0, 0, 0, 0, 0};
+    int kernel_steps_ = 0;
+};
+
+}  // namespace snowmpm

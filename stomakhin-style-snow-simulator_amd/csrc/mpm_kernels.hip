@@ -1,0 +1,1000 @@
+// mpm_kernels.hip -- hand-written CDNA4 (gfx950) kernels of the MPM step.
+//
+// One step is the reference's step() (src/grid.cpp:359-450) restructured for
+// the GPU:
+//   k_begin     reset per-pass counters
+//   k_bin       home block of each particle -> device hash, per-cell counts
+//   k_blocks    per home block: cell prefix sums, particle range, neighbour blocks
+//   k_scatter   counting-sort scatter of particle indices by (block, cell)
+//   k_cellsort  order each cell by particle id (makes P2G order canonical)
+//   k_p2g       P2G of mass, momentum and stress force (grid.cpp:119-129,
+//               162-182), LDS staged, 64-bit fixed-point global atomics
+//   k_grid      grid velocity + grid collisions (grid.cpp:195-243) fused
+//   k_g2p       stress_update + G2P + particle collisions + advection
+//               (grid.cpp:245-338) fused, also producing the next step's
+//               force matrix -V0 P F_E^T so P2G needs no SVD
+//   k_end       clear the hash slots used this pass, publish the particle count
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.hpp"
+#include "mpm_kernels.hpp"
+#include "mpm_math.hpp"
+
+namespace snowmpm {
+
+// ---------------------------------------------------------------------------
+// Device hash (open addressing, linear probing, CAS insert).  The slot index
+// is the block's storage index.
+__device__ __forceinline__ uint32_t hash_slot(uint64_t k, uint32_t mask) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (uint32_t)k & mask;
+}
+
+__device__ __forceinline__ uint32_t hash_insert(const GridDev& g, uint64_t key, bool& inserted) {
+    uint32_t h = hash_slot(key, g.mask);
+    inserted = false;
+    for (uint32_t p = 0; p < g.max_probe; ++p) {
+        uint64_t cur = __hip_atomic_load(&g.keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == key) return h;
+        if (cur == kEmptyKey) {
+            unsigned long long old = atomicCAS((unsigned long long*)&g.keys[h], (unsigned long long)kEmptyKey,
+                                               (unsigned long long)key);
+            if (old == kEmptyKey) {
+                inserted = true;
+                return h;
+            }
+            if (old == key) return h;
+        }
+        h = (h + 1) & g.mask;
+    }
+    return kInvalid;
+}
+
+__device__ __forceinline__ void flag(const StatusDev& st, uint32_t sid, uint32_t bits) {
+    atomicOr(&st.scene_status[sid], bits);
+}
+
+__device__ __forceinline__ bool finite3(float a, float b, float c) {
+    return isfinite(a) && isfinite(b) && isfinite(c);
+}
+
+// Base node of the particle's stencil, floor(x/dx) - 1 (grid.cpp:64-67).
+__device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1, float x2, int b[3], float f[3]) {
+    const float g0 = x0 * sc.inv_dx, g1 = x1 * sc.inv_dx, g2 = x2 * sc.inv_dx;
+    const float c0 = floorf(g0), c1 = floorf(g1), c2 = floorf(g2);
+    f[0] = g0 - c0;
+    f[1] = g1 - c1;
+    f[2] = g2 - c2;
+    b[0] = (int)c0 - 1;
+    b[1] = (int)c1 - 1;
+    b[2] = (int)c2 - 1;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void k_begin(StepArgs a) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        a.cnt->n_home = 0;
+        a.cnt->n_all = 0;
+        a.cnt->cursor = 0;
+        a.cnt->n_fallback = 0;
+    }
+}
+
+__global__ void k_set_count(Counters* c, uint32_t n) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->n_particles = n;
+}
+
+// Per particle: hash-insert the home block, count the particle into its cell.
+__global__ __launch_bounds__(256) void k_bin(StepArgs a) {
+    const uint32_t n = a.cnt->n_particles;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t sid = a.cur.sid[i];
+        const SceneDev& sc = a.scenes[sid];
+        uint32_t key32 = kInvalid, r = 0;
+        if (sc.flags & a.pass_flag) {
+            const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
+            if (!finite3(x0, x1, x2)) {
+                flag(a.st, sid, kStatusNaN);
+            } else {
+                int b[3];
+                float f[3];
+                base_node(sc, x0, x1, x2, b, f);
+                const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
+                const int lim = kCoordBias - 2;
+                if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
+                    flag(a.st, sid, kStatusRange);
+                } else {
+                    const uint32_t l = (uint32_t)(b[0] & 3) | ((uint32_t)(b[1] & 3) << 2) | ((uint32_t)(b[2] & 3) << 4);
+                    bool ins;
+                    const uint32_t slot = hash_insert(a.g, pack_key(sid, bx, by, bz), ins);
+                    if (slot == kInvalid) {
+                        flag(a.st, sid, kStatusHashFull);
+                    } else {
+                        if (ins) {
+                            a.g.home_list[atomicAdd(&a.cnt->n_home, 1u)] = slot;
+                            a.g.all_list[atomicAdd(&a.cnt->n_all, 1u)] = slot;
+                        }
+                        r = atomicAdd(&a.g.cellcnt[(size_t)slot * 64 + l], 1u);
+                        key32 = (slot << 6) | l;
+                    }
+                }
+            }
+        }
+        a.bin.pkey[i] = key32;
+        a.bin.prank[i] = r;
+    }
+}
+
+// Per home block (one wave): exclusive scan of the 64 cell counts, particle
+// range, and the 8 blocks its nodes live in (the block itself + {0,1}^3).
+__global__ __launch_bounds__(256) void k_blocks(StepArgs a) {
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t nh = a.cnt->n_home;
+    for (uint32_t b = blockIdx.x * 4 + wid; b < nh; b += gridDim.x * 4) {
+        const uint32_t slot = a.g.home_list[b];
+        const uint32_t c = a.g.cellcnt[(size_t)slot * 64 + lane];
+        uint32_t s = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(s, o, 64);
+            if (lane >= (uint32_t)o) s += t;
+        }
+        a.g.celloff[(size_t)slot * 64 + lane] = s - c;
+        const uint32_t total = __shfl(s, 63, 64);
+        if (lane == 0) {
+            a.g.blkoff[slot] = atomicAdd(&a.cnt->cursor, total);
+            a.g.blkcnt[slot] = total;
+        }
+        if (lane < 8) {
+            uint32_t ns = slot;
+            if (lane != 0) {
+                const uint64_t key = a.g.keys[slot];
+                uint32_t sid;
+                int bx, by, bz;
+                unpack_key(key, sid, bx, by, bz);
+                bool ins;
+                ns = hash_insert(a.g, pack_key(sid, bx + (int)(lane & 1), by + (int)((lane >> 1) & 1), bz + (int)(lane >> 2)), ins);
+                if (ns == kInvalid) {
+                    flag(a.st, sid, kStatusHashFull);
+                    ns = slot;  // keep indices in range; the pass is flagged as failed
+                } else if (ins) {
+                    a.g.all_list[atomicAdd(&a.cnt->n_all, 1u)] = ns;
+                }
+            }
+            a.g.nb[(size_t)slot * 8 + lane] = ns;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(StepArgs a) {
+    const uint32_t n = a.cnt->n_particles;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t key32 = a.bin.pkey[i];
+        if (key32 == kInvalid) continue;
+        const uint32_t slot = key32 >> 6, l = key32 & 63;
+        const uint32_t dst = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + l] + a.bin.prank[i];
+        a.bin.sidx[dst] = i;
+        a.bin.spid[dst] = a.cur.pid[i];
+    }
+}
+
+// Insertion sort of each cell's segment by particle id: the P2G summation
+// order inside a cell then depends only on the particles, not on the order
+// in which atomics handed out ranks.
+__global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t nh = a.cnt->n_home;
+    for (uint32_t b = blockIdx.x * 4 + wid; b < nh; b += gridDim.x * 4) {
+        const uint32_t slot = a.g.home_list[b];
+        const uint32_t cnt = a.g.cellcnt[(size_t)slot * 64 + lane];
+        const uint32_t s0 = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + lane];
+        for (uint32_t k = 1; k < cnt; ++k) {
+            const uint32_t kp = a.bin.spid[s0 + k], ki = a.bin.sidx[s0 + k];
+            uint32_t j = k;
+            while (j > 0 && a.bin.spid[s0 + j - 1] > kp) {
+                a.bin.spid[s0 + j] = a.bin.spid[s0 + j - 1];
+                a.bin.sidx[s0 + j] = a.bin.sidx[s0 + j - 1];
+                --j;
+            }
+            a.bin.spid[s0 + j] = kp;
+            a.bin.sidx[s0 + j] = ki;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// P2G.  One workgroup per home block, 4 waves.  Wave w owns the cell layer
+// z = w of the block; lane (cx, cy, d) owns cell (cx, cy, w) and the node
+// plane z = w + d of that cell's 4x4x4 stencil.  A lane walks its cell's
+// particles (in particle-id order) and keeps the 16 nodes x 7 channels of its
+// plane in registers, so the LDS sees one deposit per cell instead of one per
+// particle.  Deposits go to the wave's private LDS region; for a fixed loop
+// index every lane of a wave hits a different node, and a wave's LDS ops
+// retire in order, so the float sums are deterministic.  The four wave
+// regions are then summed in fixed order, converted to 64-bit fixed point
+// and added to the owning blocks with integer atomics (order independent).
+constexpr int kP2GZS = 60;                 // z stride (floats): bank-conflict-free deposits
+constexpr int kP2GCH = 4 * kP2GZS;         // one channel of one wave region
+constexpr int kP2GW = kGridCh * kP2GCH;    // one wave region
+
+__global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
+    __shared__ float lds[4 * kP2GW];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
+    const uint32_t nh = a.cnt->n_home;
+    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
+        for (int t = threadIdx.x; t < 4 * kP2GW; t += 256) lds[t] = 0.0f;
+        __syncthreads();
+        const uint32_t slot = a.g.home_list[blk];
+        uint32_t sid;
+        int bx, by, bz;
+        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        const SceneDev& sc = a.scenes[sid];
+        const float inv_dx = sc.inv_dx;
+        const int l = cx + 4 * cy + 16 * w;
+        const uint32_t cnt = a.g.cellcnt[(size_t)slot * 64 + l];
+        const uint32_t start = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + l];
+        // cell index = base node + 1
+        const float cxf = (float)(4 * bx + cx + 1), cyf = (float)(4 * by + cy + 1), czf = (float)(4 * bz + w + 1);
+
+        float acc[4][4][kGridCh];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int c = 0; c < kGridCh; ++c) acc[i][j][c] = 0.0f;
+
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t i = a.bin.sidx[start + k];
+            const float fx = a.cur.x[0][i] * inv_dx - cxf;
+            const float fy = a.cur.x[1][i] * inv_dx - cyf;
+            const float fz = a.cur.x[2][i] * inv_dx - czf;
+            const float m = a.cur.m[i];
+            const float v0 = a.cur.v[0][i], v1 = a.cur.v[1][i], v2 = a.cur.v[2][i];
+            // force matrix (symmetric) scaled to world-unit gradients
+            const float s00 = a.cur.S[0][i] * inv_dx, s11 = a.cur.S[1][i] * inv_dx, s22 = a.cur.S[2][i] * inv_dx;
+            const float s01 = a.cur.S[3][i] * inv_dx, s02 = a.cur.S[4][i] * inv_dx, s12 = a.cur.S[5][i] * inv_dx;
+            float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+            bspline(fx, wx, dwx);
+            bspline(fy, wy, dwy);
+            bspline(fz, wz, dwz);
+            float wzd = wz[0], dwzd = dwz[0];
+            if (d == 1) { wzd = wz[1]; dwzd = dwz[1]; }
+            if (d == 2) { wzd = wz[2]; dwzd = dwz[2]; }
+            if (d == 3) { wzd = wz[3]; dwzd = dwz[3]; }
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb) {
+                const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
+                const float mw = m * yz;
+                const float p0 = mw * v0, p1 = mw * v1, p2 = mw * v2;
+                const float B0 = s00 * yz, B1 = s01 * yz, B2 = s02 * yz;
+                const float A0 = s01 * dyz + s02 * ydz;
+                const float A1 = s11 * dyz + s12 * ydz;
+                const float A2 = s12 * dyz + s22 * ydz;
+#pragma unroll
+                for (int aa = 0; aa < 4; ++aa) {
+                    const float wa = wx[aa], da = dwx[aa];
+                    acc[aa][bb][0] += wa * mw;
+                    acc[aa][bb][1] += wa * p0;
+                    acc[aa][bb][2] += wa * p1;
+                    acc[aa][bb][3] += wa * p2;
+                    acc[aa][bb][4] += da * B0 + wa * A0;
+                    acc[aa][bb][5] += da * B1 + wa * A1;
+                    acc[aa][bb][6] += da * B2 + wa * A2;
+                }
+            }
+        }
+        float* region = lds + w * kP2GW;
+#pragma unroll
+        for (int aa = 0; aa < 4; ++aa)
+#pragma unroll
+            for (int bb = 0; bb < 4; ++bb)
+#pragma unroll
+                for (int c = 0; c < kGridCh; ++c)
+                    atomicAdd(&region[c * kP2GCH + d * kP2GZS + (cy + bb) * 8 + (cx + aa)], acc[aa][bb][c]);
+        __syncthreads();
+
+        // Flush the 7x7x7 node region into the 8 owning blocks.
+        for (int t = threadIdx.x; t < 8 * kGridCh * 64; t += 256) {
+            const int tgt = t / (kGridCh * 64);
+            const int rem = t - tgt * (kGridCh * 64);
+            const int c = rem >> 6, nn = rem & 63;
+            const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
+                      Z = (tgt >> 2) * 4 + (nn >> 4);
+            if (X > 6 || Y > 6 || Z > 6) continue;
+            float val = 0.0f;
+#pragma unroll
+            for (int ww = 0; ww < 4; ++ww) {
+                const int zl = Z - ww;
+                if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + c * kP2GCH + zl * kP2GZS + Y * 8 + X];
+            }
+            if (val != 0.0f) {
+                const float scale = c == 0 ? sc.scale_m : (c < 4 ? sc.scale_p : sc.scale_f);
+                double q = (double)val * (double)scale;
+                if (!(fabs(q) < 4.0e18)) {
+                    flag(a.st, sid, isfinite(q) ? kStatusOverflow : kStatusNaN);
+                    q = 0.0;
+                }
+                const long long qi = __double2ll_rn(q);
+                const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
+                atomicAdd(&a.g.acc[((size_t)ts * kGridCh + c) * 64 + nn], (unsigned long long)qi);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Grid update: grid_velocity (grid.cpp:195-204) then grid_collisions
+// (grid.cpp:233-243), in FP64, plus the PIC/FLIP blend u that G2P needs:
+// u = (1-alpha) v + alpha [m>0] (v - p/m)   (grid.cpp:312-324).
+// One wave per block; consumes and clears the fixed-point accumulators.
+__global__ __launch_bounds__(256) void k_grid(StepArgs a) {
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t na = a.cnt->n_all;
+    for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
+        const uint32_t slot = a.g.all_list[b];
+        uint32_t sid;
+        int bx, by, bz;
+        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        const SceneDev& sc = a.scenes[sid];
+        unsigned long long* acc = a.g.acc + (size_t)slot * kGridCh * 64;
+        long long q[kGridCh];
+#pragma unroll
+        for (int c = 0; c < kGridCh; ++c) {
+            q[c] = (long long)acc[c * 64 + lane];
+            acc[c * 64 + lane] = 0ull;
+        }
+        double v[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
+        if (a.mass_only == 0 && q[0] > 0) {
+            const double m = (double)q[0] * sc.inv_scale_m;
+            const double im = 1.0 / m;
+            const double dt = sc.dt;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double p = (double)q[1 + r] * sc.inv_scale_p;
+                const double f = (double)q[4 + r] * sc.inv_scale_f;
+                const double vold = p * im;
+                v[r] = vold + dt * (f * im + (double)sc.gravity[r]);
+                dv[r] = -vold;
+            }
+        }
+        if (a.mass_only == 0) {
+            const double dx = sc.dx;
+            const double xn0 = dx * (double)(4 * bx + (int)(lane & 3));
+            const double xn1 = dx * (double)(4 * by + (int)((lane >> 2) & 3));
+            const double xn2 = dx * (double)(4 * bz + (int)(lane >> 4));
+            collide<double>(sc, xn0, xn1, xn2, v[0], v[1], v[2]);
+            const double al = sc.alpha;
+            float* out = a.g.vu + (size_t)slot * kVelCh * 64;
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const double flip = (q[0] > 0) ? (v[r] + dv[r]) : 0.0;
+                out[r * 64 + lane] = (float)v[r];
+                out[(3 + r) * 64 + lane] = (float)((1.0 - al) * v[r] + al * flip);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plasticity (grid.cpp:277-285) and the next step's stress
+// (grid.cpp:162-176) from one factorisation.  The trial elastic gradient
+// Fh = F_E + dt L F_E is factored through the symmetric eigenproblem
+// Fh^T Fh = V diag(s^2) V^T.  Clamping s to [1-theta_c, 1+theta_s] gives
+// F_E' = U diag(s_c) V^T = Fh V diag(s_c/s) V^T, J_P' = J_P / prod(s_c/s),
+// F_P' = V diag(s/s_c) V^T F_P, and, with U = Fh V diag(1/s),
+// -V0 P(F_E') F_E'^T = -V0 U diag(2 mu (s_c-1) s_c + lambda (J-1) J) U^T.
+// Inverted, singular or badly conditioned trial gradients (cond > kappa) are
+// redone in FP64 with an explicit unsigned SVD.
+struct PlastOut {
+    float Fe[9];
+    float Jp;
+    float S[6];
+};
+
+__device__ __noinline__ void plastic_fp64(const float Fe[9], const float L[9], float dt, float Jp_old,
+                                          const float* Fp_old, float* Fp_new, bool clampit, float tc, float ts,
+                                          float mu0, float la0, float xi, float V0, PlastOut& o) {
+    double F[9], Fh[9];
+    for (int i = 0; i < 9; ++i) F[i] = Fe[i];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += (double)L[r * 3 + k] * F[k * 3 + c];
+            Fh[r * 3 + c] = F[r * 3 + c] + (double)dt * s;
+        }
+    double U[9], S[3], V[9];
+    svd3_fp64(Fh, U, S, V);
+    double sc[3];
+    for (int k = 0; k < 3; ++k) {
+        sc[k] = S[k];
+        if (clampit) sc[k] = fmin(fmax(S[k], 1.0 - (double)tc), 1.0 + (double)ts);
+    }
+    double Fn[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < 3; ++k) s += U[r * 3 + k] * sc[k] * V[c * 3 + k];
+            Fn[r * 3 + c] = s;
+        }
+    const double detFh = det3<double>(Fh), detFn = det3<double>(Fn);
+    double Jp = (double)Jp_old * (detFn != 0.0 ? detFh / detFn : 0.0);
+    if (Fp_old) {
+        // F_P' = F_E'^{-1} Fh F_P   (grid.cpp:285)
+        double inv[9];
+        const double id = 1.0 / detFn;
+        inv[0] = (Fn[4] * Fn[8] - Fn[5] * Fn[7]) * id;
+        inv[1] = (Fn[2] * Fn[7] - Fn[1] * Fn[8]) * id;
+        inv[2] = (Fn[1] * Fn[5] - Fn[2] * Fn[4]) * id;
+        inv[3] = (Fn[5] * Fn[6] - Fn[3] * Fn[8]) * id;
+        inv[4] = (Fn[0] * Fn[8] - Fn[2] * Fn[6]) * id;
+        inv[5] = (Fn[2] * Fn[3] - Fn[0] * Fn[5]) * id;
+        inv[6] = (Fn[3] * Fn[7] - Fn[4] * Fn[6]) * id;
+        inv[7] = (Fn[1] * Fn[6] - Fn[0] * Fn[7]) * id;
+        inv[8] = (Fn[0] * Fn[4] - Fn[1] * Fn[3]) * id;
+        double T[9], P[9];
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += inv[r * 3 + k] * Fh[k * 3 + c];
+                T[r * 3 + c] = s;
+            }
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k) s += T[r * 3 + k] * (double)Fp_old[k * 3 + c];
+                P[r * 3 + c] = s;
+            }
+        for (int i = 0; i < 9; ++i) Fp_new[i] = (float)P[i];
+        Jp = det3<double>(P);
+    }
+    const double h = exp((double)xi * (1.0 - Jp));
+    const double mu = (double)mu0 * h, la = (double)la0 * h;
+    const double J = detFn;
+    double q[3];
+    for (int k = 0; k < 3; ++k) q[k] = 2.0 * mu * (sc[k] - 1.0) * sc[k] + la * (J - 1.0) * J;
+    const int rr[6] = {0, 1, 2, 0, 0, 1}, cc[6] = {0, 1, 2, 1, 2, 2};
+    for (int e = 0; e < 6; ++e) {
+        double s = 0.0;
+        for (int k = 0; k < 3; ++k) s += q[k] * U[rr[e] * 3 + k] * U[cc[e] * 3 + k];
+        o.S[e] = (float)(-(double)V0 * s);
+    }
+    for (int i = 0; i < 9; ++i) o.Fe[i] = (float)Fn[i];
+    o.Jp = (float)Jp;
+}
+
+// Returns true when the FP64 path was (or, with defer, must be) taken.
+template <bool TRACK>
+__device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float (&L)[9], float dt, float Jp_old,
+                                               const float (&Fp_old)[9], float (&Fp_new)[9], bool clampit,
+                                               float tc, float ts, float mu0, float la0, float xi, float V0,
+                                               float kappa, PlastOut& o, bool defer = false) {
+    float Fh[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float s = L[r * 3 + 0] * Fe[0 * 3 + c] + L[r * 3 + 1] * Fe[1 * 3 + c] + L[r * 3 + 2] * Fe[2 * 3 + c];
+            Fh[r * 3 + c] = Fe[r * 3 + c] + dt * s;
+        }
+    float A[3][3], V[3][3], lam[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = r; c < 3; ++c) {
+            const float s = Fh[0 * 3 + r] * Fh[0 * 3 + c] + Fh[1 * 3 + r] * Fh[1 * 3 + c] + Fh[2 * 3 + r] * Fh[2 * 3 + c];
+            A[r][c] = s;
+            A[c][r] = s;
+        }
+    sym_eig3<float, 4>(A, V, lam);
+    float s[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = sqrtf(fmaxf(lam[k], 0.0f));
+    const float detFh = det3<float>(Fh);
+    const float smin = fminf(s[0], fminf(s[1], s[2])), smax = fmaxf(s[0], fmaxf(s[1], s[2]));
+    if (!(detFh > 0.0f) || !(smin > 0.0f) || !(smax <= kappa * smin)) {
+        if (!defer) plastic_fp64(Fe, L, dt, Jp_old, TRACK ? Fp_old : nullptr, Fp_new, clampit, tc, ts, mu0, la0, xi, V0, o);
+        return true;
+    }
+    float sc[3], cr[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        sc[k] = clampit ? fminf(fmaxf(s[k], 1.0f - tc), 1.0f + ts) : s[k];
+        cr[k] = sc[k] / s[k];
+    }
+    const bool plastic = (cr[0] != 1.0f) || (cr[1] != 1.0f) || (cr[2] != 1.0f);
+    const float prodc = cr[0] * cr[1] * cr[2];
+    if (plastic) {
+        float W[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                W[r][c] = cr[0] * V[r][0] * V[c][0] + cr[1] * V[r][1] * V[c][1] + cr[2] * V[r][2] * V[c][2];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                o.Fe[r * 3 + c] = Fh[r * 3 + 0] * W[0][c] + Fh[r * 3 + 1] * W[1][c] + Fh[r * 3 + 2] * W[2][c];
+        if (TRACK) {
+            float Wi[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    Wi[r][c] = V[r][0] * V[c][0] / cr[0] + V[r][1] * V[c][1] / cr[1] + V[r][2] * V[c][2] / cr[2];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    Fp_new[r * 3 + c] = Wi[r][0] * Fp_old[0 * 3 + c] + Wi[r][1] * Fp_old[1 * 3 + c] + Wi[r][2] * Fp_old[2 * 3 + c];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) o.Fe[i] = Fh[i];
+        if (TRACK) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Fp_new[i] = Fp_old[i];
+        }
+    }
+    o.Jp = Jp_old / prodc;
+    const float J = detFh * prodc;
+    const float h = expf(xi * (1.0f - o.Jp));
+    const float mu = mu0 * h, la = la0 * h;
+    float U[3][3], q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float is = 1.0f / s[k];
+#pragma unroll
+        for (int r = 0; r < 3; ++r) U[r][k] = (Fh[r * 3 + 0] * V[0][k] + Fh[r * 3 + 1] * V[1][k] + Fh[r * 3 + 2] * V[2][k]) * is;
+        q[k] = 2.0f * mu * (sc[k] - 1.0f) * sc[k] + la * (J - 1.0f) * J;
+    }
+    const float nv = -V0;
+    o.S[0] = nv * (q[0] * U[0][0] * U[0][0] + q[1] * U[0][1] * U[0][1] + q[2] * U[0][2] * U[0][2]);
+    o.S[1] = nv * (q[0] * U[1][0] * U[1][0] + q[1] * U[1][1] * U[1][1] + q[2] * U[1][2] * U[1][2]);
+    o.S[2] = nv * (q[0] * U[2][0] * U[2][0] + q[1] * U[2][1] * U[2][1] + q[2] * U[2][2] * U[2][2]);
+    o.S[3] = nv * (q[0] * U[0][0] * U[1][0] + q[1] * U[0][1] * U[1][1] + q[2] * U[0][2] * U[1][2]);
+    o.S[4] = nv * (q[0] * U[0][0] * U[2][0] + q[1] * U[0][1] * U[2][1] + q[2] * U[0][2] * U[2][2]);
+    o.S[5] = nv * (q[0] * U[1][0] * U[2][0] + q[1] * U[1][1] * U[2][1] + q[2] * U[1][2] * U[2][2]);
+    return false;
+}
+
+__device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
+    if (sc.use_E) {
+        const float E = p.E[i], nu = sc.nu;
+        mu0 = E / (2.0f * (1.0f + nu));
+        la0 = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+    } else {
+        mu0 = sc.mu0;
+        la0 = sc.lambda0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused stress_update + G2P + particle collisions + advection.  One workgroup
+// per home block stages the 8x8x8 nodes of its 8 blocks (v and u) in LDS,
+// then each thread updates one particle and writes it to the next buffer at
+// its sorted position.
+template <bool TRACK>
+__global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
+    __shared__ float vu[kVelCh * 512];
+    const uint32_t nh = a.cnt->n_home;
+    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
+        const uint32_t slot = a.g.home_list[blk];
+        uint32_t sid;
+        int bx, by, bz;
+        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        const SceneDev& sc = a.scenes[sid];
+        for (int t = threadIdx.x; t < 8 * kVelCh * 64; t += 256) {
+            const int tgt = t / (kVelCh * 64);
+            const int rem = t - tgt * (kVelCh * 64);
+            const int c = rem >> 6, nn = rem & 63;
+            const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
+            const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
+                      Z = (tgt >> 2) * 4 + (nn >> 4);
+            vu[c * 512 + Z * 64 + Y * 8 + X] = a.g.vu[((size_t)ts * kVelCh + c) * 64 + nn];
+        }
+        if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
+        const float inv_dx = sc.inv_dx, dt = sc.dt, alpha = sc.alpha;
+        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += 256) {
+            const uint32_t i = a.bin.sidx[j];
+            const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
+            float v0 = a.cur.v[0][i], v1 = a.cur.v[1][i], v2 = a.cur.v[2][i];
+            int b[3];
+            float f[3];
+            base_node(sc, x0, x1, x2, b, f);
+            int lx = b[0] - 4 * bx, ly = b[1] - 4 * by, lz = b[2] - 4 * bz;
+            if ((unsigned)lx > 3u || (unsigned)ly > 3u || (unsigned)lz > 3u) {
+                flag(a.st, sid, kStatusRange);
+                lx = min(max(lx, 0), 3);
+                ly = min(max(ly, 0), 3);
+                lz = min(max(lz, 0), 3);
+            }
+            float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+            bspline(f[0], wx, dwx);
+            bspline(f[1], wy, dwy);
+            bspline(f[2], wz, dwz);
+            float pv0 = 0.f, pv1 = 0.f, pv2 = 0.f;
+            float L[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) L[e] = 0.f;
+#pragma unroll 1
+            for (int dd = 0; dd < 4; ++dd) {
+                const float wzd = dd == 0 ? wz[0] : (dd == 1 ? wz[1] : (dd == 2 ? wz[2] : wz[3]));
+                const float dwzd = dd == 0 ? dwz[0] : (dd == 1 ? dwz[1] : (dd == 2 ? dwz[2] : dwz[3]));
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const int rowb = (lz + dd) * 64 + (ly + bb) * 8 + lx;
+                    float S0 = 0.f, S1 = 0.f, S2 = 0.f, X0 = 0.f, X1 = 0.f, X2 = 0.f, U0 = 0.f, U1 = 0.f, U2 = 0.f;
+#pragma unroll
+                    for (int aa = 0; aa < 4; ++aa) {
+                        const float gv0 = vu[0 * 512 + rowb + aa], gv1 = vu[1 * 512 + rowb + aa], gv2 = vu[2 * 512 + rowb + aa];
+                        const float gu0 = vu[3 * 512 + rowb + aa], gu1 = vu[4 * 512 + rowb + aa], gu2 = vu[5 * 512 + rowb + aa];
+                        S0 += wx[aa] * gv0;
+                        S1 += wx[aa] * gv1;
+                        S2 += wx[aa] * gv2;
+                        X0 += dwx[aa] * gv0;
+                        X1 += dwx[aa] * gv1;
+                        X2 += dwx[aa] * gv2;
+                        U0 += wx[aa] * gu0;
+                        U1 += wx[aa] * gu1;
+                        U2 += wx[aa] * gu2;
+                    }
+                    const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
+                    pv0 += yz * U0;
+                    pv1 += yz * U1;
+                    pv2 += yz * U2;
+                    L[0] += yz * X0;
+                    L[1] += dyz * S0;
+                    L[2] += ydz * S0;
+                    L[3] += yz * X1;
+                    L[4] += dyz * S1;
+                    L[5] += ydz * S1;
+                    L[6] += yz * X2;
+                    L[7] += dyz * S2;
+                    L[8] += ydz * S2;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 9; ++e) L[e] *= inv_dx;
+
+            float Fe[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) Fe[e] = a.cur.F[e][i];
+            const float Jp = a.cur.Jp[i];
+            const float V0 = a.cur.V0[i];
+            float mu0, la0;
+            lame(sc, a.cur, i, mu0, la0);
+            float Fp_old[9], Fp_new[9];
+            constexpr bool track = TRACK;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) Fp_old[e] = TRACK ? a.cur.Fp[e][i] : 0.f;
+            PlastOut po;
+            const bool fb = plastic_update<TRACK>(Fe, L, dt, Jp, Fp_old, Fp_new, true, sc.theta_c, sc.theta_s, mu0,
+                                                  la0, sc.hardening, V0, a.kappa, po, true);
+            if (fb) {
+                // Redone in FP64 by k_fp64_fixup; keep finite placeholders meanwhile.
+                const uint32_t q = atomicAdd(&a.cnt->n_fallback, 1u);
+                a.fb.pos[q] = j;
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    a.fb.L[(size_t)q * 9 + e] = L[e];
+                    po.Fe[e] = Fe[e];
+                }
+                po.Jp = Jp;
+#pragma unroll
+                for (int e = 0; e < 6; ++e) po.S[e] = 0.f;
+                if (track)
+#pragma unroll
+                    for (int e = 0; e < 9; ++e) Fp_new[e] = Fp_old[e];
+            }
+
+            // v_p = (1-alpha) v_pic + alpha (v_p + sum w (v_i - p_i/m_i))  (grid.cpp:321-324)
+            v0 = alpha * v0 + pv0;
+            v1 = alpha * v1 + pv1;
+            v2 = alpha * v2 + pv2;
+            collide<float>(sc, x0, x1, x2, v0, v1, v2);           // grid.cpp:327-333, pre-advection x
+            const float nx0 = x0 + dt * v0, nx1 = x1 + dt * v1, nx2 = x2 + dt * v2;  // grid.cpp:335-338
+            if (!finite3(nx0, nx1, nx2) || !finite3(v0, v1, v2) || !isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
+
+            a.nxt.x[0][j] = nx0;
+            a.nxt.x[1][j] = nx1;
+            a.nxt.x[2][j] = nx2;
+            a.nxt.v[0][j] = v0;
+            a.nxt.v[1][j] = v1;
+            a.nxt.v[2][j] = v2;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) a.nxt.F[e][j] = po.Fe[e];
+            a.nxt.Jp[j] = po.Jp;
+            if (track) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) a.nxt.Fp[e][j] = Fp_new[e];
+            }
+#pragma unroll
+            for (int e = 0; e < 6; ++e) a.nxt.S[e][j] = po.S[e];
+            a.nxt.m[j] = a.cur.m[i];
+            a.nxt.V0[j] = V0;
+            if (a.copy_E) a.nxt.E[j] = a.cur.E[i];
+            a.nxt.pid[j] = a.cur.pid[i];
+            a.nxt.sid[j] = sid;
+        }
+        __syncthreads();
+    }
+}
+
+// FP64 redo of the plasticity update for the particles G2P deferred
+// (inverted, singular or cond(trial F_E) > kappa).  Reads the pre-step
+// state through the sorted index, writes the post-step state in place.
+__global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
+    const uint32_t nf = a.cnt->n_fallback;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x) {
+        const uint32_t j = a.fb.pos[q];
+        const uint32_t i = a.bin.sidx[j];
+        const uint32_t sid = a.cur.sid[i];
+        const SceneDev& sc = a.scenes[sid];
+        float Fe[9], L[9], Fp_old[9], Fp_new[9];
+        for (int e = 0; e < 9; ++e) {
+            Fe[e] = a.cur.F[e][i];
+            L[e] = a.fb.L[(size_t)q * 9 + e];
+        }
+        const bool track = a.cur.Fp[0] != nullptr;
+        if (track)
+            for (int e = 0; e < 9; ++e) Fp_old[e] = a.cur.Fp[e][i];
+        float mu0, la0;
+        lame(sc, a.cur, i, mu0, la0);
+        PlastOut po;
+        plastic_fp64(Fe, L, sc.dt, a.cur.Jp[i], track ? Fp_old : nullptr, Fp_new, true, sc.theta_c, sc.theta_s, mu0,
+                     la0, sc.hardening, a.cur.V0[i], po);
+        for (int e = 0; e < 9; ++e) a.nxt.F[e][j] = po.Fe[e];
+        a.nxt.Jp[j] = po.Jp;
+        for (int e = 0; e < 6; ++e) a.nxt.S[e][j] = po.S[e];
+        if (track)
+            for (int e = 0; e < 9; ++e) a.nxt.Fp[e][j] = Fp_new[e];
+        if (!isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
+        atomicAdd(&a.st.fp64_count[sid], 1ull);
+    }
+}
+
+// Clear the hash slots used this pass; publish the new particle count.
+__global__ __launch_bounds__(256) void k_end(StepArgs a, int publish) {
+    const uint32_t na = a.cnt->n_all;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < na; b += gridDim.x * blockDim.x)
+        a.g.keys[a.g.all_list[b]] = kEmptyKey;
+    if (publish && blockIdx.x == 0 && threadIdx.x == 0) a.cnt->n_particles = a.cnt->cursor;
+}
+
+// ---------------------------------------------------------------------------
+// Setup (grid.cpp:340-357): after a mass-only P2G, rho_p = sum_i m_i w_ip / dx^3
+// and V_p0 = m_p / rho_p (calculate_volume, grid.cpp:138-160); then the
+// particle's force matrix for the first step.  Writes in place.
+__global__ __launch_bounds__(256) void k_setup_gather(StepArgs a) {
+    __shared__ float ms[512];
+    const uint32_t nh = a.cnt->n_home;
+    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
+        const uint32_t slot = a.g.home_list[blk];
+        uint32_t sid;
+        int bx, by, bz;
+        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        const SceneDev& sc = a.scenes[sid];
+        for (int t = threadIdx.x; t < 8 * 64; t += 256) {
+            const int tgt = t >> 6, nn = t & 63;
+            const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
+            const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
+                      Z = (tgt >> 2) * 4 + (nn >> 4);
+            const long long q = (long long)a.g.acc[((size_t)ts * kGridCh + 0) * 64 + nn];
+            ms[Z * 64 + Y * 8 + X] = (float)((double)q * sc.inv_scale_m);
+        }
+        if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
+        __syncthreads();
+        const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
+        const float idx3 = sc.inv_dx * sc.inv_dx * sc.inv_dx;
+        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += 256) {
+            const uint32_t i = a.bin.sidx[j];
+            int b[3];
+            float f[3];
+            base_node(sc, a.cur.x[0][i], a.cur.x[1][i], a.cur.x[2][i], b, f);
+            const int lx = min(max(b[0] - 4 * bx, 0), 3), ly = min(max(b[1] - 4 * by, 0), 3),
+                      lz = min(max(b[2] - 4 * bz, 0), 3);
+            float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+            bspline(f[0], wx, dwx);
+            bspline(f[1], wy, dwy);
+            bspline(f[2], wz, dwz);
+            float rho = 0.f;
+#pragma unroll
+            for (int dd = 0; dd < 4; ++dd)
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const int rowb = (lz + dd) * 64 + (ly + bb) * 8 + lx;
+                    float s = 0.f;
+#pragma unroll
+                    for (int aa = 0; aa < 4; ++aa) s += wx[aa] * ms[rowb + aa];
+                    rho += wy[bb] * wz[dd] * s;
+                }
+            rho *= idx3;
+            const float V0 = a.cur.m[i] / rho;
+            a.cur.V0[i] = V0;
+            float Fe[9], L[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) {
+                Fe[e] = a.cur.F[e][i];
+                L[e] = 0.f;
+            }
+            float mu0, la0;
+            lame(sc, a.cur, i, mu0, la0);
+            PlastOut po;
+            float dummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dummy2[9];
+            plastic_update<false>(Fe, L, 0.f, a.cur.Jp[i], dummy, dummy2, false, 0.f, 0.f, mu0, la0, sc.hardening,
+                                  V0, a.kappa, po);
+#pragma unroll
+            for (int e = 0; e < 6; ++e) a.cur.S[e][i] = po.S[e];
+        }
+        __syncthreads();
+    }
+}
+
+// Clears the accumulators of every block used by a mass-only pass.
+__global__ __launch_bounds__(256) void k_clear_acc(StepArgs a) {
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint32_t na = a.cnt->n_all;
+    for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
+        const uint32_t slot = a.g.all_list[b];
+#pragma unroll
+        for (int c = 0; c < kGridCh; ++c) a.g.acc[((size_t)slot * kGridCh + c) * 64 + lane] = 0ull;
+    }
+}
+
+// Force matrix from the stored F_E (force_constitutive, grid.cpp:162-176) for
+// particles [first, first+n) of the current buffer (after an upload).
+__global__ __launch_bounds__(256) void k_init_stress(StepArgs a, uint32_t first, uint32_t n) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        const uint32_t i = first + t;
+        const SceneDev& sc = a.scenes[a.cur.sid[i]];
+        float Fe[9], L[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            Fe[e] = a.cur.F[e][i];
+            L[e] = 0.f;
+        }
+        float mu0, la0;
+        lame(sc, a.cur, i, mu0, la0);
+        PlastOut po;
+        float dummy[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, dummy2[9];
+        plastic_update<false>(Fe, L, 0.f, a.cur.Jp[i], dummy, dummy2, false, 0.f, 0.f, mu0, la0, sc.hardening,
+                              a.cur.V0[i], a.kappa, po);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) a.cur.S[e][i] = po.S[e];
+    }
+}
+
+// Gathers scene `sid`'s particles into pid order (download staging).
+__global__ __launch_bounds__(256) void k_gather_scene(StepArgs a, uint32_t sid, float* out, uint32_t nscene,
+                                                      int track) {
+    const uint32_t n = a.cnt->n_particles;
+    // out layout: x3 v3 F9 Jp V0 [Fp9], each block of nscene floats
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (a.cur.sid[i] != sid) continue;
+        const uint32_t p = a.cur.pid[i];
+        if (p >= nscene) continue;
+        size_t o = 0;
+        for (int e = 0; e < 3; ++e, o += nscene) out[o + p] = a.cur.x[e][i];
+        for (int e = 0; e < 3; ++e, o += nscene) out[o + p] = a.cur.v[e][i];
+        for (int e = 0; e < 9; ++e, o += nscene) out[o + p] = a.cur.F[e][i];
+        out[o + p] = a.cur.Jp[i];
+        o += nscene;
+        out[o + p] = a.cur.V0[i];
+        o += nscene;
+        if (track)
+            for (int e = 0; e < 9; ++e, o += nscene) out[o + p] = a.cur.Fp[e][i];
+    }
+}
+
+// FP64 conservation sums of one scene: sum m, sum m x (3), sum m v (3).
+__global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, double* out7) {
+    __shared__ double red[7][256];
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    const uint32_t n = a.cnt->n_particles;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (a.cur.sid[i] != sid) continue;
+        const double m = a.cur.m[i];
+        s[0] += m;
+        for (int e = 0; e < 3; ++e) {
+            s[1 + e] += m * (double)a.cur.x[e][i];
+            s[4 + e] += m * (double)a.cur.v[e][i];
+        }
+    }
+    for (int e = 0; e < 7; ++e) red[e][threadIdx.x] = s[e];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int e = 0; e < 7; ++e) red[e][threadIdx.x] += red[e][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int e = 0; e < 7; ++e) out7[blockIdx.x * 7 + e] = red[e][0];
+}
+
+// Per-scene maximum of a positive quantity (float bits compare as uints).
+__global__ __launch_bounds__(256) void k_scene_max(StepArgs a, int what) {
+    const uint32_t n = a.cnt->n_particles;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t sid = a.cur.sid[i];
+        const SceneDev& sc = a.scenes[sid];
+        float v = 0.f;
+        if (what == 0) {
+            float mu0, la0;
+            lame(sc, a.cur, i, mu0, la0);
+            v = a.cur.V0[i] * (mu0 + la0);
+        }
+        if (v > 0.f && isfinite(v)) atomicMax((unsigned int*)&a.st.vmax[sid], __float_as_uint(v));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launch wrappers.
+
+void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
+    if (ev) (void)hipEventRecord(ev[0], s);
+    k_begin<<<1, 64, 0, s>>>(a);
+    k_bin<<<c.grid_p, 256, 0, s>>>(a);
+    k_blocks<<<c.grid_b, 256, 0, s>>>(a);
+    k_scatter<<<c.grid_p, 256, 0, s>>>(a);
+    k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    k_p2g<<<c.grid_w, 256, 0, s>>>(a);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    k_grid<<<c.grid_b, 256, 0, s>>>(a);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    if (a.cur.Fp[0])
+        k_g2p<true><<<c.grid_w, 256, 0, s>>>(a);
+    else
+        k_g2p<false><<<c.grid_w, 256, 0, s>>>(a);
+    k_fp64_fixup<<<c.grid_b, 256, 0, s>>>(a);
+    if (ev) (void)hipEventRecord(ev[4], s);
+    k_end<<<c.grid_b, 256, 0, s>>>(a, 1);
+    if (ev) (void)hipEventRecord(ev[5], s);
+}
+
+void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    k_begin<<<1, 64, 0, s>>>(a);
+    k_bin<<<c.grid_p, 256, 0, s>>>(a);
+    k_blocks<<<c.grid_b, 256, 0, s>>>(a);
+    k_scatter<<<c.grid_p, 256, 0, s>>>(a);
+    k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
+    k_p2g<<<c.grid_w, 256, 0, s>>>(a);
+    k_setup_gather<<<c.grid_w, 256, 0, s>>>(a);
+    k_clear_acc<<<c.grid_b, 256, 0, s>>>(a);
+    k_end<<<c.grid_b, 256, 0, s>>>(a, 0);
+}
+
+void launch_init_stress(const StepArgs& a, uint32_t first, uint32_t n, hipStream_t s) {
+    if (n == 0) return;
+    const uint32_t g = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+    k_init_stress<<<g, 256, 0, s>>>(a, first, n);
+}
+
+void launch_set_count(Counters* c, uint32_t n, hipStream_t s) { k_set_count<<<1, 64, 0, s>>>(c, n); }
+
+void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, float* out, uint32_t nscene, int track,
+                         hipStream_t s) {
+    k_gather_scene<<<c.grid_p, 256, 0, s>>>(a, sid, out, nscene, track);
+}
+
+void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s) {
+    k_scene_sums<<<nblocks, 256, 0, s>>>(a, sid, out);
+}
+
+void launch_scene_max(const StepArgs& a, const LaunchCfg& c, int what, hipStream_t s) {
+    k_scene_max<<<c.grid_p, 256, 0, s>>>(a, what);
+}
+
+}  // namespace snowmpm

@@ -1,0 +1,44 @@
+// mpm_kernels.hpp -- kernel argument block and host launch wrappers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "mpm_device.hpp"
+
+namespace snowmpm {
+
+struct StepArgs {
+    PBuf cur;             // particles read this pass
+    PBuf nxt;             // particles written by G2P (sorted order)
+    GridDev g;
+    BinDev bin;
+    StatusDev st;
+    FallbackDev fb;
+    Counters* cnt;
+    const SceneDev* scenes;
+    uint32_t pass_flag;   // kSceneActive for steps, kSceneSetup for setup
+    int mass_only;
+    int copy_E;           // some scene carries per-particle E
+    float kappa;          // FP64 fallback bound on cond(trial F_E)
+};
+
+struct LaunchCfg {
+    int grid_p;  // per-particle grid-stride kernels
+    int grid_b;  // one wave per block kernels (4 blocks per workgroup)
+    int grid_w;  // one workgroup per block kernels (P2G, G2P)
+};
+
+// Kernel slots timed by launch_step's events:
+// [0,1) binning+sort, [1,2) P2G, [2,3) grid, [3,4) G2P, [4,5) end.
+constexpr int kStepEvents = 6;
+
+void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev);
+void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
+void launch_init_stress(const StepArgs& a, uint32_t first, uint32_t n, hipStream_t s);
+void launch_set_count(Counters* c, uint32_t n, hipStream_t s);
+void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, float* out, uint32_t nscene, int track,
+                         hipStream_t s);
+void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s);
+void launch_scene_max(const StepArgs& a, const LaunchCfg& c, int what, hipStream_t s);
+
+}  // namespace snowmpm

@@ -1,0 +1,286 @@
+"""snowmpm -- Python host binding of the MI355X MPM step engine (libsnowmpm.so).
+
+A thin ctypes layer over the C ABI in include/snowmpm.h.  It mirrors the
+reference's solver interface (include/snow/grid.hpp:96-97,
+include/snow/collision.hpp:14, include/snow/debug.hpp:3-15): particles are
+numpy arrays in the reference's per-particle layout (x, v: n x 3; F_E, F_P:
+n x 3 x 3), ``setup`` computes V_p0, ``step`` advances every resident scene,
+``make_plane`` builds the reference's collider.
+
+The library is required: importing this module fails loudly when
+libsnowmpm.so is missing, and every call runs on the GPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libsnowmpm.so")
+if not os.path.exists(_LIB):
+    raise ImportError(f"libsnowmpm.so not built ({_LIB}); run __graft_entry__.build()")
+lib = C.CDLL(_LIB)
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_IO = -1, -2, -3, -4, -5, -6, -7
+STATUS_NAN, STATUS_OVERFLOW, STATUS_HASH_FULL, STATUS_RANGE = 0x1, 0x2, 0x4, 0x8
+PLANE, BOX, SPHERE = 0, 1, 2
+
+
+class MpmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """Scalar arguments of the reference step() (grid.hpp:97) + nu."""
+    _fields_ = [("gravity", C.c_double * 3), ("dt", C.c_double), ("dx", C.c_double),
+                ("mu0", C.c_double), ("lambda0", C.c_double), ("hardening", C.c_double),
+                ("max_compression", C.c_double), ("max_stretch", C.c_double),
+                ("alpha", C.c_double), ("nu", C.c_double)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["gravity"] = list(self.gravity)
+        return d
+
+
+class Collider(C.Structure):
+    _fields_ = [("type", C.c_int), ("pad_", C.c_int), ("point", C.c_double * 3),
+                ("normal", C.c_double * 3), ("half_extent", C.c_double * 3),
+                ("radius", C.c_double), ("velocity", C.c_double * 3), ("friction", C.c_double)]
+
+
+class StepTimings(C.Structure):
+    """StepTimings (debug.hpp:3-15), milliseconds accumulated with +=."""
+    _fields_ = [(n, C.c_double) for n in (
+        "stencil", "p2g", "forces_constitutive", "forces_accumulative", "grid_velocity",
+        "grid_collisions", "stress_update", "g2p", "particle_collisions", "position_update")]
+
+
+class Config(C.Structure):
+    _fields_ = [("particle_capacity", C.c_size_t), ("hash_log2", C.c_int), ("track_fp", C.c_int),
+                ("device", C.c_int), ("pad_", C.c_int), ("kappa", C.c_double)]
+
+
+_pd = C.POINTER(C.c_double)
+_ppd = C.POINTER(_pd)
+_SIGS = {
+    "mpm_create": (C.c_int, [C.POINTER(Config), C.POINTER(C.c_void_p)]),
+    "mpm_destroy": (C.c_int, [C.c_void_p]),
+    "mpm_last_error": (C.c_char_p, []),
+    "mpm_scene_add": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Collider), C.c_int, C.c_size_t,
+                                _pd, _pd, _pd, _pd, _pd, _pd, _pd, C.POINTER(C.c_int)]),
+    "mpm_scene_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_scene_count": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "mpm_setup": (C.c_int, [C.c_void_p]),
+    "mpm_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(StepTimings)]),
+    "mpm_scene_download": (C.c_int, [C.c_void_p, C.c_int, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
+    "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
+    "mpm_make_plane": (Collider, [_pd, _pd, C.c_double]),
+    "mpm_make_box": (Collider, [_pd, _pd, C.c_double]),
+    "mpm_make_sphere": (Collider, [_pd, C.c_double, C.c_double]),
+    "mpm_import_particles": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_size_t), _ppd, _ppd, _ppd]),
+    "mpm_export_frame": (C.c_int, [C.c_char_p, C.c_int, C.c_size_t, _pd]),
+    "mpm_free": (None, [C.c_void_p]),
+    "mpm_scene_generate": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_double, C.POINTER(C.c_size_t),
+                                     _ppd, _ppd, _ppd, _ppd, C.POINTER(Params), C.POINTER(Collider),
+                                     C.POINTER(C.c_int)]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+# Symbols declared in include/snowmpm.h (checked by the ABI test).
+EXPORTED = tuple(_SIGS)
+
+
+def last_error():
+    return lib.mpm_last_error().decode(errors="replace")
+
+
+def _check(rc):
+    if rc != OK:
+        raise MpmError(rc, last_error())
+    return rc
+
+
+def _dptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_pd)
+
+
+def _as_f64(a, shape):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape)
+
+
+def _v3(x):
+    return (C.c_double * 3)(*map(float, x))
+
+
+def make_plane(point, normal, friction):
+    """make_plane (collision.hpp:14, collision.cpp:5-27)."""
+    return lib.mpm_make_plane(_v3(point), _v3(normal), float(friction))
+
+
+def make_box(centre, half_extent, friction):
+    return lib.mpm_make_box(_v3(centre), _v3(half_extent), float(friction))
+
+
+def make_sphere(centre, radius, friction):
+    return lib.mpm_make_sphere(_v3(centre), float(radius), float(friction))
+
+
+def lame(E, nu):
+    """mu_0, lambda_0 from E and nu (main.cpp:28-29)."""
+    return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
+
+
+def params(dt, dx, mu0, lambda0, hardening, max_compression, max_stretch, alpha, gravity=(0, -9.81, 0), nu=0.2):
+    p = Params()
+    p.gravity[:] = list(map(float, gravity))
+    p.dt, p.dx, p.mu0, p.lambda0 = float(dt), float(dx), float(mu0), float(lambda0)
+    p.hardening, p.max_compression, p.max_stretch = float(hardening), float(max_compression), float(max_stretch)
+    p.alpha, p.nu = float(alpha), float(nu)
+    return p
+
+
+def _take(ptr, n, cols):
+    if not ptr:
+        return None
+    a = np.ctypeslib.as_array(ptr, shape=(n * cols,)).copy()
+    lib.mpm_free(C.cast(ptr, C.c_void_p))
+    return a.reshape(n, cols) if cols > 1 else a
+
+
+def generate_scene(kind, seed=20261203, index=0, scale=1.0):
+    """Seeded workload: kind 0 cube, 1 two tori, 2 slab, 3 batched-distribution scene."""
+    n = C.c_size_t()
+    x, v, m, E = _pd(), _pd(), _pd(), _pd()
+    prm = Params()
+    coll = (Collider * 8)()
+    nc = C.c_int()
+    _check(lib.mpm_scene_generate(int(kind), C.c_uint64(seed), int(index), float(scale), C.byref(n),
+                                  C.byref(x), C.byref(v), C.byref(m), C.byref(E), C.byref(prm), coll,
+                                  C.byref(nc)))
+    k = n.value
+    out = dict(x=_take(x, k, 3), v=_take(v, k, 3), m=_take(m, k, 1), E=_take(E, k, 1) if E else None,
+               params=prm, colliders=[coll[i] for i in range(nc.value)])
+    return out
+
+
+def import_particles(groups_csv, particles_csv):
+    """import_particles (import.hpp:25, import.cpp:131-137) -> (x, v, m)."""
+    n = C.c_size_t()
+    x, v, m = _pd(), _pd(), _pd()
+    _check(lib.mpm_import_particles(os.fsencode(groups_csv), os.fsencode(particles_csv), C.byref(n),
+                                    C.byref(x), C.byref(v), C.byref(m)))
+    k = n.value
+    return _take(x, k, 3), _take(v, k, 3), _take(m, k, 1)
+
+
+def export_frame(directory, frame_no, x):
+    """export_particles (export.cpp:10-22) into directory/frame_%06d.csv."""
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    _check(lib.mpm_export_frame(os.fsencode(directory), int(frame_no), x.shape[0], _dptr(x)))
+
+
+class Engine:
+    """Device-resident MPM engine holding any number of scenes."""
+
+    def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0):
+        cfg = Config(particle_capacity=int(capacity), hash_log2=int(hash_log2), track_fp=int(bool(track_fp)),
+                     device=int(device), pad_=0, kappa=float(kappa))
+        h = C.c_void_p()
+        _check(lib.mpm_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.track_fp = bool(track_fp)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.mpm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_scene(self, prm, colliders, x, v, m, V0=None, Fe=None, Fp=None, E=None):
+        x = _as_f64(x, (-1, 3))
+        n = x.shape[0]
+        v = _as_f64(v, (n, 3))
+        m = _as_f64(np.broadcast_to(np.asarray(m, dtype=np.float64), (n,)), (n,))
+        V0 = None if V0 is None else _as_f64(np.broadcast_to(np.asarray(V0, dtype=np.float64), (n,)), (n,))
+        Fe = _as_f64(Fe, (n, 9))
+        Fp = _as_f64(Fp, (n, 9))
+        E = None if E is None else _as_f64(np.broadcast_to(np.asarray(E, dtype=np.float64), (n,)), (n,))
+        arr = (Collider * max(1, len(colliders)))(*colliders)
+        sid = C.c_int()
+        _check(lib.mpm_scene_add(self.h, C.byref(prm), arr, len(colliders), n, _dptr(x), _dptr(v), _dptr(m),
+                                 _dptr(V0), _dptr(Fe), _dptr(Fp), _dptr(E), C.byref(sid)))
+        return sid.value
+
+    def remove_scene(self, sid):
+        _check(lib.mpm_scene_remove(self.h, int(sid)))
+
+    def count(self, sid):
+        n = C.c_size_t()
+        _check(lib.mpm_scene_count(self.h, int(sid), C.byref(n)))
+        return n.value
+
+    def setup(self):
+        """setup() (grid.cpp:340-357) for every scene added without V0."""
+        _check(lib.mpm_setup(self.h))
+
+    def step(self, nsteps=1, timings=None):
+        """nsteps x step() (grid.cpp:359-450) on every active scene."""
+        return _check(lib.mpm_step(self.h, int(nsteps), C.byref(timings) if timings is not None else None))
+
+    def step_rc(self, nsteps=1):
+        """Like step() but returns the status code instead of raising."""
+        return lib.mpm_step(self.h, int(nsteps), None)
+
+    def download(self, sid):
+        n = self.count(sid)
+        x = np.zeros((n, 3))
+        v = np.zeros((n, 3))
+        Fe = np.zeros((n, 9))
+        Jp = np.zeros(n)
+        V0 = np.zeros(n)
+        Fp = np.zeros((n, 9)) if self.track_fp else None
+        _check(lib.mpm_scene_download(self.h, int(sid), _dptr(x), _dptr(v), _dptr(Fe), _dptr(Fp), _dptr(Jp),
+                                      _dptr(V0)))
+        out = dict(x=x, v=v, Fe=Fe.reshape(n, 3, 3), Jp=Jp, V0=V0)
+        if Fp is not None:
+            out["Fp"] = Fp.reshape(n, 3, 3)
+        return out
+
+    def status(self, sid):
+        st = C.c_uint32()
+        fb = C.c_uint64()
+        _check(lib.mpm_scene_status(self.h, int(sid), C.byref(st), C.byref(fb)))
+        return st.value, fb.value
+
+    def sums(self, sid):
+        out = (C.c_double * 7)()
+        _check(lib.mpm_scene_sums(self.h, int(sid), out))
+        return np.array(out[:])
+
+    def kernel_times(self):
+        ms = (C.c_double * 5)()
+        n = C.c_int()
+        _check(lib.mpm_kernel_times(self.h, ms, C.byref(n)))
+        return dict(bin_sort=ms[0], p2g=ms[1], grid=ms[2], g2p=ms[3], end=ms[4], steps=n.value)
