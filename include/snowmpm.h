@@ -108,6 +108,8 @@ int mpm_scene_add(mpm_engine* e, const mpm_params* prm, const mpm_collider* coll
                   const double* x, const double* v, const double* m, const double* V0, const double* Fe,
                   const double* Fp, const double* E, int* scene_id);
 int mpm_scene_remove(mpm_engine* e, int scene);
+/* Drop every scene and empty the particle buffer at once. */
+int mpm_clear(mpm_engine* e);
 int mpm_scene_count(mpm_engine* e, int scene, size_t* n);
 
 /* setup(): V_p0 for every scene added without V0 (grid.cpp:340-357). */

@@ -345,6 +345,20 @@ int Engine::scene_remove(int sid) {
     return MPM_OK;
 }
 
+int Engine::clear() {
+    for (size_t s = 0; s < scenes_.size(); ++s) {
+        h_scenes_[s].flags = 0;
+        int rc = upload_scene_table((int)s);
+        if (rc) return rc;
+    }
+    scenes_.clear();
+    any_E_ = false;
+    launch_set_count(cnt_, 0, stream_);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return MPM_OK;
+}
+
 int Engine::setup() {
     bool any = false;
     for (size_t s = 0; s < scenes_.size(); ++s)
@@ -554,6 +568,8 @@ int mpm_scene_add(mpm_engine* e, const mpm_params* prm, const mpm_collider* coll
 }
 
 int mpm_scene_remove(mpm_engine* e, int scene) { return e ? e->eng.scene_remove(scene) : MPM_ERR_ARG; }
+
+int mpm_clear(mpm_engine* e) { return e ? e->eng.clear() : MPM_ERR_ARG; }
 
 int mpm_scene_count(mpm_engine* e, int scene, size_t* n) {
     if (!e || !n) return MPM_ERR_ARG;

@@ -34,6 +34,7 @@ class Engine {
                   const double* v, const double* m, const double* V0, const double* Fe, const double* Fp,
                   const double* E, int* sid);
     int scene_remove(int sid);
+    int clear();
     int scene_count(int sid, size_t* n) const {
         if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) return MPM_ERR_ARG;
         *n = scenes_[sid].n;

@@ -73,6 +73,7 @@ _SIGS = {
     "mpm_scene_add": (C.c_int, [C.c_void_p, C.POINTER(Params), C.POINTER(Collider), C.c_int, C.c_size_t,
                                 _pd, _pd, _pd, _pd, _pd, _pd, _pd, C.POINTER(C.c_int)]),
     "mpm_scene_remove": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_clear": (C.c_int, [C.c_void_p]),
     "mpm_scene_count": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]),
     "mpm_setup": (C.c_int, [C.c_void_p]),
     "mpm_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(StepTimings)]),
@@ -235,6 +236,9 @@ class Engine:
 
     def remove_scene(self, sid):
         _check(lib.mpm_scene_remove(self.h, int(sid)))
+
+    def clear(self):
+        _check(lib.mpm_clear(self.h))
 
     def count(self, sid):
         n = C.c_size_t()
