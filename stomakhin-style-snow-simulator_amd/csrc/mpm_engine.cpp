@@ -91,6 +91,8 @@ int Engine::init(const mpm_config& cfg) {
         return MPM_ERR_ARG;
     }
     H_ = 1u << hl;
+    launch_.grid_l = (int)std::max<uint32_t>(1u, H_ / 4096u);
+    launch_.grid_k = cus * 2;
     const bool track = cfg_.track_fp != 0;
 
     int rc;

@@ -89,85 +89,184 @@ __global__ void k_set_count(Counters* c, uint32_t n) {
 }
 
 // Per particle: hash-insert the home block, count the particle into its cell.
+// Consecutive particles mostly share a block and a cell (the buffer is sorted
+// by the previous step), so both are aggregated per wave: one lane inserts
+// each distinct block key, one lane bumps each distinct cell counter by the
+// number of lanes in it, and lanes take consecutive ranks.
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t sid = a.cur.sid[i];
-        const SceneDev& sc = a.scenes[sid];
-        uint32_t key32 = kInvalid, r = 0;
-        if (sc.flags & a.pass_flag) {
-            const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
-            if (!finite3(x0, x1, x2)) {
-                flag(a.st, sid, kStatusNaN);
-            } else {
-                int b[3];
-                float f[3];
-                base_node(sc, x0, x1, x2, b, f);
-                const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
-                const int lim = kCoordBias - 2;
-                if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
-                    flag(a.st, sid, kStatusRange);
+    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
+        const uint32_t i = wbase + lane;
+        uint64_t key = kEmptyKey;
+        uint32_t l = 0, sid = 0;
+        bool want = false;
+        if (i < n) {
+            sid = a.cur.sid[i];
+            const SceneDev& sc = a.scenes[sid];
+            if (sc.flags & a.pass_flag) {
+                const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
+                if (!finite3(x0, x1, x2)) {
+                    flag(a.st, sid, kStatusNaN);
                 } else {
-                    const uint32_t l = (uint32_t)(b[0] & 3) | ((uint32_t)(b[1] & 3) << 2) | ((uint32_t)(b[2] & 3) << 4);
-                    bool ins;
-                    const uint32_t slot = hash_insert(a.g, pack_key(sid, bx, by, bz), ins);
-                    if (slot == kInvalid) {
-                        flag(a.st, sid, kStatusHashFull);
+                    int b[3];
+                    float f[3];
+                    base_node(sc, x0, x1, x2, b, f);
+                    const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
+                    const int lim = kCoordBias - 2;
+                    if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
+                        flag(a.st, sid, kStatusRange);
                     } else {
-                        if (ins) {
-                            a.g.home_list[atomicAdd(&a.cnt->n_home, 1u)] = slot;
-                            a.g.all_list[atomicAdd(&a.cnt->n_all, 1u)] = slot;
-                        }
-                        r = atomicAdd(&a.g.cellcnt[(size_t)slot * 64 + l], 1u);
-                        key32 = (slot << 6) | l;
+                        l = (uint32_t)(b[0] & 3) | ((uint32_t)(b[1] & 3) << 2) | ((uint32_t)(b[2] & 3) << 4);
+                        key = pack_key(sid, bx, by, bz);
+                        want = true;
                     }
                 }
             }
         }
-        a.bin.pkey[i] = key32;
-        a.bin.prank[i] = r;
+        // one hash insert per distinct block key in the wave
+        uint32_t slot = kInvalid;
+        uint64_t pending = __ballot(want);
+        while (pending) {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t klo = __shfl((uint32_t)key, leader), khi = __shfl((uint32_t)(key >> 32), leader);
+            const uint64_t lkey = ((uint64_t)khi << 32) | klo;
+            const bool match = want && key == lkey;
+            uint32_t s_ = 0;
+            if ((int)lane == leader) {
+                bool ins;
+                s_ = hash_insert(a.g, lkey, ins);
+                if (s_ == kInvalid) flag(a.st, sid, kStatusHashFull);
+            }
+            s_ = __shfl(s_, leader);
+            if (match) slot = s_;
+            pending &= ~__ballot(match);
+        }
+        // one counter update per distinct cell in the wave
+        const bool have = slot != kInvalid;
+        const uint32_t ck = have ? (slot << 6) | l : kInvalid;
+        uint32_t r = 0;
+        uint64_t pend2 = __ballot(have);
+        while (pend2) {
+            const int leader = __ffsll((unsigned long long)pend2) - 1;
+            const uint32_t lck = __shfl(ck, leader);
+            const bool match = have && ck == lck;
+            const uint64_t mm = __ballot(match);
+            uint32_t basecnt = 0;
+            if ((int)lane == leader) basecnt = atomicAdd(&a.g.cellcnt[lck], (uint32_t)__popcll(mm));
+            basecnt = __shfl(basecnt, leader);
+            if (match) r = basecnt + (uint32_t)__popcll(mm & lt_mask);
+            pend2 &= ~mm;
+        }
+        if (i < n) {
+            a.bin.pkey[i] = ck;
+            a.bin.prank[i] = r;
+        }
+    }
+}
+
+// Compacts the occupied hash slots into a list (after k_bin: the home
+// blocks; after k_blocks: every block).  One atomic per 4096-slot tile.
+__global__ __launch_bounds__(1024) void k_list(StepArgs a, int all) {
+    __shared__ uint32_t wsum[16];
+    __shared__ uint32_t base_s;
+    uint32_t* list = all ? a.g.all_list : a.g.home_list;
+    uint32_t* counter = all ? &a.cnt->n_all : &a.cnt->n_home;
+    const uint32_t H = a.g.mask + 1;
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (uint32_t tile = blockIdx.x * 4096u; tile < H; tile += gridDim.x * 4096u) {
+        const uint32_t s0 = tile + threadIdx.x * 4;
+        uint32_t occ = 0, cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (s0 + k < H && a.g.keys[s0 + k] != kEmptyKey) {
+                occ |= 1u << k;
+                ++cnt;
+            }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o, 64);
+            if (lane >= (uint32_t)o) incl += t;
+        }
+        if (lane == 63) wsum[wid] = incl;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t v = wsum[k];
+                wsum[k] = run;
+                run += v;
+            }
+            base_s = run ? atomicAdd(counter, run) : 0u;
+        }
+        __syncthreads();
+        uint32_t pos = base_s + wsum[wid] + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (occ & (1u << k)) list[pos++] = s0 + k;
+        __syncthreads();
     }
 }
 
 // Per home block (one wave): exclusive scan of the 64 cell counts, particle
 // range, and the 8 blocks its nodes live in (the block itself + {0,1}^3).
-__global__ __launch_bounds__(256) void k_blocks(StepArgs a) {
+// A workgroup of 16 waves takes one particle range for its 16 blocks.
+__global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
+    __shared__ uint32_t tot[16];
+    __shared__ uint32_t base_s;
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t b = blockIdx.x * 4 + wid; b < nh; b += gridDim.x * 4) {
-        const uint32_t slot = a.g.home_list[b];
-        const uint32_t c = a.g.cellcnt[(size_t)slot * 64 + lane];
-        uint32_t s = c;
+    for (uint32_t b0 = blockIdx.x * 16; b0 < nh; b0 += gridDim.x * 16) {
+        const uint32_t b = b0 + wid;
+        const bool valid = b < nh;
+        const uint32_t slot = valid ? a.g.home_list[b] : 0u;
+        uint32_t total = 0;
+        if (valid) {
+            const uint32_t c = a.g.cellcnt[(size_t)slot * 64 + lane];
+            uint32_t s = c;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(s, o, 64);
-            if (lane >= (uint32_t)o) s += t;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(s, o, 64);
+                if (lane >= (uint32_t)o) s += t;
+            }
+            a.g.celloff[(size_t)slot * 64 + lane] = s - c;
+            total = __shfl(s, 63, 64);
+            if (lane < 8) {
+                uint32_t ns = slot;
+                if (lane != 0) {
+                    uint32_t sid;
+                    int bx, by, bz;
+                    unpack_key(a.g.keys[slot], sid, bx, by, bz);
+                    bool ins;
+                    ns = hash_insert(a.g, pack_key(sid, bx + (int)(lane & 1), by + (int)((lane >> 1) & 1), bz + (int)(lane >> 2)), ins);
+                    if (ns == kInvalid) {
+                        flag(a.st, sid, kStatusHashFull);
+                        ns = slot;  // keep indices in range; the pass is flagged as failed
+                    }
+                }
+                a.g.nb[(size_t)slot * 8 + lane] = ns;
+            }
         }
-        a.g.celloff[(size_t)slot * 64 + lane] = s - c;
-        const uint32_t total = __shfl(s, 63, 64);
-        if (lane == 0) {
-            a.g.blkoff[slot] = atomicAdd(&a.cnt->cursor, total);
+        if (lane == 0) tot[wid] = total;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t run = 0;
+            for (int k = 0; k < 16; ++k) {
+                const uint32_t v = tot[k];
+                tot[k] = run;
+                run += v;
+            }
+            base_s = run ? atomicAdd(&a.cnt->cursor, run) : 0u;
+        }
+        __syncthreads();
+        if (valid && lane == 0) {
+            a.g.blkoff[slot] = base_s + tot[wid];
             a.g.blkcnt[slot] = total;
         }
-        if (lane < 8) {
-            uint32_t ns = slot;
-            if (lane != 0) {
-                const uint64_t key = a.g.keys[slot];
-                uint32_t sid;
-                int bx, by, bz;
-                unpack_key(key, sid, bx, by, bz);
-                bool ins;
-                ns = hash_insert(a.g, pack_key(sid, bx + (int)(lane & 1), by + (int)((lane >> 1) & 1), bz + (int)(lane >> 2)), ins);
-                if (ns == kInvalid) {
-                    flag(a.st, sid, kStatusHashFull);
-                    ns = slot;  // keep indices in range; the pass is flagged as failed
-                } else if (ins) {
-                    a.g.all_list[atomicAdd(&a.cnt->n_all, 1u)] = ns;
-                }
-            }
-            a.g.nb[(size_t)slot * 8 + lane] = ns;
-        }
+        __syncthreads();
     }
 }
 
@@ -209,28 +308,35 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// P2G.  One workgroup per home block, 4 waves.  Wave w owns the cell layer
-// z = w of the block; lane (cx, cy, d) owns cell (cx, cy, w) and the node
-// plane z = w + d of that cell's 4x4x4 stencil.  A lane walks its cell's
-// particles (in particle-id order) and keeps the 16 nodes x 7 channels of its
-// plane in registers, so the LDS sees one deposit per cell instead of one per
+// P2G.  One workgroup per home block, 4 waves.  The block's particles (in
+// sorted order: cell, then particle id) are first staged into LDS by all 256
+// threads with coalesced loads, already reduced to what P2G needs: the offset
+// in the cell f, m, m v and the force matrix scaled by 1/dx.  Then wave w owns
+// the cell layer z = w of the block; lane (cx, cy, d) owns cell (cx, cy, w)
+// and the node plane z = w + d of that cell's 4x4x4 stencil.  A lane walks its
+// cell's particles and keeps the 16 nodes x 7 channels of its plane in
+// registers, so the LDS sees one deposit per cell instead of one per
 // particle.  Deposits go to the wave's private LDS region; for a fixed loop
 // index every lane of a wave hits a different node, and a wave's LDS ops
 // retire in order, so the float sums are deterministic.  The four wave
 // regions are then summed in fixed order, converted to 64-bit fixed point
 // and added to the owning blocks with integer atomics (order independent).
-constexpr int kP2GZS = 60;                 // z stride (floats): bank-conflict-free deposits
-constexpr int kP2GCH = 4 * kP2GZS;         // one channel of one wave region
-constexpr int kP2GW = kGridCh * kP2GCH;    // one wave region
+constexpr int kP2GNS = 12;                 // floats per node in a deposit region (7 used; 12 spreads banks)
+constexpr int kP2GW = 4 * 49 * kP2GNS;     // one wave region: 4 z-planes x 7 x 7 nodes
+constexpr int kP2GChunk = 512;             // particles staged per pass
+constexpr int kP2GStride = kP2GChunk + kP2GChunk / 32;  // per-field stride after the k + k/32 swizzle
+constexpr int kP2GFields = 13;             // fx fy fz m mv0 mv1 mv2 s00 s11 s22 s01 s02 s12
+
+__device__ __forceinline__ int p2g_swz(int k) { return k + (k >> 5); }
 
 __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
-    __shared__ float lds[4 * kP2GW];
+    __shared__ __align__(16) float lds[4 * kP2GW];
+    __shared__ float pst[kP2GFields * kP2GStride];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
     const uint32_t nh = a.cnt->n_home;
     for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
         for (int t = threadIdx.x; t < 4 * kP2GW; t += 256) lds[t] = 0.0f;
-        __syncthreads();
         const uint32_t slot = a.g.home_list[blk];
         uint32_t sid;
         int bx, by, bz;
@@ -238,10 +344,10 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
         const SceneDev& sc = a.scenes[sid];
         const float inv_dx = sc.inv_dx;
         const int l = cx + 4 * cy + 16 * w;
-        const uint32_t cnt = a.g.cellcnt[(size_t)slot * 64 + l];
-        const uint32_t start = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + l];
-        // cell index = base node + 1
-        const float cxf = (float)(4 * bx + cx + 1), cyf = (float)(4 * by + cy + 1), czf = (float)(4 * bz + w + 1);
+        const int cbeg = (int)a.g.celloff[(size_t)slot * 64 + l];
+        const int cend = cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + l];
+        const uint32_t off = a.g.blkoff[slot];
+        const int nb = (int)a.g.blkcnt[slot];
 
         float acc[4][4][kGridCh];
 #pragma unroll
@@ -251,54 +357,87 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
 #pragma unroll
                 for (int c = 0; c < kGridCh; ++c) acc[i][j][c] = 0.0f;
 
-        for (uint32_t k = 0; k < cnt; ++k) {
-            const uint32_t i = a.bin.sidx[start + k];
-            const float fx = a.cur.x[0][i] * inv_dx - cxf;
-            const float fy = a.cur.x[1][i] * inv_dx - cyf;
-            const float fz = a.cur.x[2][i] * inv_dx - czf;
-            const float m = a.cur.m[i];
-            const float v0 = a.cur.v[0][i], v1 = a.cur.v[1][i], v2 = a.cur.v[2][i];
-            // force matrix (symmetric) scaled to world-unit gradients
-            const float s00 = a.cur.S[0][i] * inv_dx, s11 = a.cur.S[1][i] * inv_dx, s22 = a.cur.S[2][i] * inv_dx;
-            const float s01 = a.cur.S[3][i] * inv_dx, s02 = a.cur.S[4][i] * inv_dx, s12 = a.cur.S[5][i] * inv_dx;
-            float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
-            bspline(fx, wx, dwx);
-            bspline(fy, wy, dwy);
-            bspline(fz, wz, dwz);
-            float wzd = wz[0], dwzd = dwz[0];
-            if (d == 1) { wzd = wz[1]; dwzd = dwz[1]; }
-            if (d == 2) { wzd = wz[2]; dwzd = dwz[2]; }
-            if (d == 3) { wzd = wz[3]; dwzd = dwz[3]; }
+        for (int c0 = 0; c0 < nb; c0 += kP2GChunk) {
+            const int cn = min(kP2GChunk, nb - c0);
+            __syncthreads();  // previous chunk fully consumed (and LDS zeroing ordered)
+            for (int t = threadIdx.x; t < cn; t += 256) {
+                const uint32_t i = a.bin.sidx[off + c0 + t];
+                const float g0 = a.cur.x[0][i] * inv_dx, g1 = a.cur.x[1][i] * inv_dx, g2 = a.cur.x[2][i] * inv_dx;
+                const float m = a.cur.m[i];
+                const int q = p2g_swz(t);
+                pst[0 * kP2GStride + q] = g0 - floorf(g0);
+                pst[1 * kP2GStride + q] = g1 - floorf(g1);
+                pst[2 * kP2GStride + q] = g2 - floorf(g2);
+                pst[3 * kP2GStride + q] = m;
+                pst[4 * kP2GStride + q] = m * a.cur.v[0][i];
+                pst[5 * kP2GStride + q] = m * a.cur.v[1][i];
+                pst[6 * kP2GStride + q] = m * a.cur.v[2][i];
 #pragma unroll
-            for (int bb = 0; bb < 4; ++bb) {
-                const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
-                const float mw = m * yz;
-                const float p0 = mw * v0, p1 = mw * v1, p2 = mw * v2;
-                const float B0 = s00 * yz, B1 = s01 * yz, B2 = s02 * yz;
-                const float A0 = s01 * dyz + s02 * ydz;
-                const float A1 = s11 * dyz + s12 * ydz;
-                const float A2 = s12 * dyz + s22 * ydz;
+                for (int e = 0; e < 6; ++e) pst[(7 + e) * kP2GStride + q] = a.cur.S[e][i] * inv_dx;
+            }
+            __syncthreads();
+            const int kb = max(cbeg - c0, 0), ke = min(cend - c0, cn);
+            for (int k = kb; k < ke; ++k) {
+                const int q = p2g_swz(k);
+                const float fx = pst[0 * kP2GStride + q], fy = pst[1 * kP2GStride + q], fz = pst[2 * kP2GStride + q];
+                const float m = pst[3 * kP2GStride + q];
+                const float p0 = pst[4 * kP2GStride + q], p1 = pst[5 * kP2GStride + q], p2 = pst[6 * kP2GStride + q];
+                const float s00 = pst[7 * kP2GStride + q], s11 = pst[8 * kP2GStride + q], s22 = pst[9 * kP2GStride + q];
+                const float s01 = pst[10 * kP2GStride + q], s02 = pst[11 * kP2GStride + q],
+                            s12 = pst[12 * kP2GStride + q];
+                float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+                bspline(fx, wx, dwx);
+                bspline(fy, wy, dwy);
+                bspline(fz, wz, dwz);
+                float wzd = wz[0], dwzd = dwz[0];
+                if (d == 1) { wzd = wz[1]; dwzd = dwz[1]; }
+                if (d == 2) { wzd = wz[2]; dwzd = dwz[2]; }
+                if (d == 3) { wzd = wz[3]; dwzd = dwz[3]; }
 #pragma unroll
-                for (int aa = 0; aa < 4; ++aa) {
-                    const float wa = wx[aa], da = dwx[aa];
-                    acc[aa][bb][0] += wa * mw;
-                    acc[aa][bb][1] += wa * p0;
-                    acc[aa][bb][2] += wa * p1;
-                    acc[aa][bb][3] += wa * p2;
-                    acc[aa][bb][4] += da * B0 + wa * A0;
-                    acc[aa][bb][5] += da * B1 + wa * A1;
-                    acc[aa][bb][6] += da * B2 + wa * A2;
+                for (int bb = 0; bb < 4; ++bb) {
+                    const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
+                    const float mw = m * yz;
+                    const float q0 = p0 * yz, q1 = p1 * yz, q2 = p2 * yz;
+                    const float B0 = s00 * yz, B1 = s01 * yz, B2 = s02 * yz;
+                    const float A0 = s01 * dyz + s02 * ydz;
+                    const float A1 = s11 * dyz + s12 * ydz;
+                    const float A2 = s12 * dyz + s22 * ydz;
+#pragma unroll
+                    for (int aa = 0; aa < 4; ++aa) {
+                        const float wa = wx[aa], da = dwx[aa];
+                        acc[aa][bb][0] += wa * mw;
+                        acc[aa][bb][1] += wa * q0;
+                        acc[aa][bb][2] += wa * q1;
+                        acc[aa][bb][3] += wa * q2;
+                        acc[aa][bb][4] += da * B0 + wa * A0;
+                        acc[aa][bb][5] += da * B1 + wa * A1;
+                        acc[aa][bb][6] += da * B2 + wa * A2;
+                    }
                 }
             }
         }
+        // Ordered read-add-write deposits: for one (aa, bb) the 64 lanes of the
+        // wave touch 64 different nodes; the compiler barrier keeps the
+        // program order of the iterations, which the LDS preserves per wave.
         float* region = lds + w * kP2GW;
 #pragma unroll
         for (int aa = 0; aa < 4; ++aa)
 #pragma unroll
-            for (int bb = 0; bb < 4; ++bb)
-#pragma unroll
-                for (int c = 0; c < kGridCh; ++c)
-                    atomicAdd(&region[c * kP2GCH + d * kP2GZS + (cy + bb) * 8 + (cx + aa)], acc[aa][bb][c]);
+            for (int bb = 0; bb < 4; ++bb) {
+                float* np = region + ((d * 7 + cy + bb) * 7 + cx + aa) * kP2GNS;
+                float4 lo = *reinterpret_cast<float4*>(np);
+                float4 hi = *reinterpret_cast<float4*>(np + 4);
+                lo.x += acc[aa][bb][0];
+                lo.y += acc[aa][bb][1];
+                lo.z += acc[aa][bb][2];
+                lo.w += acc[aa][bb][3];
+                hi.x += acc[aa][bb][4];
+                hi.y += acc[aa][bb][5];
+                hi.z += acc[aa][bb][6];
+                *reinterpret_cast<float4*>(np) = lo;
+                *reinterpret_cast<float4*>(np + 4) = hi;
+                __asm__ __volatile__("" ::: "memory");
+            }
         __syncthreads();
 
         // Flush the 7x7x7 node region into the 8 owning blocks.
@@ -313,7 +452,7 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
 #pragma unroll
             for (int ww = 0; ww < 4; ++ww) {
                 const int zl = Z - ww;
-                if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + c * kP2GCH + zl * kP2GZS + Y * 8 + X];
+                if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
             }
             if (val != 0.0f) {
                 const float scale = c == 0 ? sc.scale_m : (c < 4 ? sc.scale_p : sc.scale_f);
@@ -923,19 +1062,44 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
         for (int e = 0; e < 7; ++e) out7[blockIdx.x * 7 + e] = red[e][0];
 }
 
-// Per-scene maximum of a positive quantity (float bits compare as uints).
+// Per-scene maximum of a positive quantity (float bits compare as uints),
+// reduced per thread (while the scene id stays the same) and per wave before
+// the atomics.
 __global__ __launch_bounds__(256) void k_scene_max(StepArgs a, int what) {
     const uint32_t n = a.cnt->n_particles;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t sid = a.cur.sid[i];
-        const SceneDev& sc = a.scenes[sid];
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t cur_sid = kInvalid;
+    float best = 0.f;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
+        const uint32_t i = wbase + lane;
+        uint32_t sid = kInvalid;
         float v = 0.f;
-        if (what == 0) {
-            float mu0, la0;
-            lame(sc, a.cur, i, mu0, la0);
-            v = a.cur.V0[i] * (mu0 + la0);
+        if (i < n) {
+            sid = a.cur.sid[i];
+            const SceneDev& sc = a.scenes[sid];
+            if (what == 0) {
+                float mu0, la0;
+                lame(sc, a.cur, i, mu0, la0);
+                v = a.cur.V0[i] * (mu0 + la0);
+            }
+            if (!(v > 0.f) || !isfinite(v)) v = 0.f;
         }
-        if (v > 0.f && isfinite(v)) atomicMax((unsigned int*)&a.st.vmax[sid], __float_as_uint(v));
+        if (sid != cur_sid) {
+            if (cur_sid != kInvalid && best > 0.f) atomicMax((unsigned int*)&a.st.vmax[cur_sid], __float_as_uint(best));
+            cur_sid = sid;
+            best = 0.f;
+        }
+        best = fmaxf(best, v);
+    }
+    // wave reduction when the whole wave holds one scene
+    const uint32_t s0 = __shfl(cur_sid, 0);
+    if (__all(cur_sid == s0)) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
+        if (lane == 0 && s0 != kInvalid && best > 0.f) atomicMax((unsigned int*)&a.st.vmax[s0], __float_as_uint(best));
+    } else if (cur_sid != kInvalid && best > 0.f) {
+        atomicMax((unsigned int*)&a.st.vmax[cur_sid], __float_as_uint(best));
     }
 }
 
@@ -946,7 +1110,9 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     if (ev) (void)hipEventRecord(ev[0], s);
     k_begin<<<1, 64, 0, s>>>(a);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
-    k_blocks<<<c.grid_b, 256, 0, s>>>(a);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a, 0);
+    k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a, 1);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -967,7 +1133,9 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
 void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_begin<<<1, 64, 0, s>>>(a);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
-    k_blocks<<<c.grid_b, 256, 0, s>>>(a);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a, 0);
+    k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a, 1);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     k_p2g<<<c.grid_w, 256, 0, s>>>(a);

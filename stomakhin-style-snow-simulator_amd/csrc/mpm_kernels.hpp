@@ -26,6 +26,8 @@ struct LaunchCfg {
     int grid_p;  // per-particle grid-stride kernels
     int grid_b;  // one wave per block kernels (4 blocks per workgroup)
     int grid_w;  // one workgroup per block kernels (P2G, G2P)
+    int grid_l;  // hash-slot compaction (4096 slots per workgroup)
+    int grid_k;  // k_blocks (16 blocks per workgroup)
 };
 
 // Kernel slots timed by launch_step's events:
