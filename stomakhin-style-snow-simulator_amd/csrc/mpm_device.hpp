@@ -44,6 +44,7 @@ struct SceneDev {
     float dx, inv_dx, dt;
     float gravity[3];
     float mu0, lambda0, nu;
+    float mu_per_E, la_per_E;                // mu0 = E mu_per_E, lambda0 = E la_per_E
     float hardening, theta_c, theta_s, alpha;
     float scale_m, scale_p, scale_f;         // float -> fixed point (powers of two)
     double inv_scale_m, inv_scale_p, inv_scale_f;

@@ -287,6 +287,8 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     sd.mu0 = (float)prm.mu0;
     sd.lambda0 = (float)prm.lambda0;
     sd.nu = (float)prm.nu;
+    sd.mu_per_E = (float)(1.0 / (2.0 * (1.0 + prm.nu)));
+    sd.la_per_E = (float)(prm.nu / ((1.0 + prm.nu) * (1.0 - 2.0 * prm.nu)));
     sd.hardening = (float)prm.hardening;
     sd.theta_c = (float)prm.max_compression;
     sd.theta_s = (float)prm.max_stretch;

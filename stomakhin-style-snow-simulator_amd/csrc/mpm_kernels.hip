@@ -636,7 +636,7 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
     sym_eig3<float, 4>(A, V, lam);
     float s[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) s[k] = sqrtf(fmaxf(lam[k], 0.0f));
+    for (int k = 0; k < 3; ++k) s[k] = sqrt_(fmaxf(lam[k], 0.0f));
     const float detFh = det3<float>(Fh);
     const float smin = fminf(s[0], fminf(s[1], s[2])), smax = fmaxf(s[0], fmaxf(s[1], s[2]));
     if (!(detFh > 0.0f) || !(smin > 0.0f) || !(smax <= kappa * smin)) {
@@ -647,7 +647,7 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         sc[k] = clampit ? fminf(fmaxf(s[k], 1.0f - tc), 1.0f + ts) : s[k];
-        cr[k] = sc[k] / s[k];
+        cr[k] = sc[k] * rcp_(s[k]);
     }
     const bool plastic = (cr[0] != 1.0f) || (cr[1] != 1.0f) || (cr[2] != 1.0f);
     const float prodc = cr[0] * cr[1] * cr[2];
@@ -669,7 +669,8 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
             for (int r = 0; r < 3; ++r)
 #pragma unroll
                 for (int c = 0; c < 3; ++c)
-                    Wi[r][c] = V[r][0] * V[c][0] / cr[0] + V[r][1] * V[c][1] / cr[1] + V[r][2] * V[c][2] / cr[2];
+                    Wi[r][c] = V[r][0] * V[c][0] * rcp_(cr[0]) + V[r][1] * V[c][1] * rcp_(cr[1]) +
+                               V[r][2] * V[c][2] * rcp_(cr[2]);
 #pragma unroll
             for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -684,14 +685,14 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
             for (int i = 0; i < 9; ++i) Fp_new[i] = Fp_old[i];
         }
     }
-    o.Jp = Jp_old / prodc;
+    o.Jp = Jp_old * rcp_(prodc);
     const float J = detFh * prodc;
     const float h = expf(xi * (1.0f - o.Jp));
     const float mu = mu0 * h, la = la0 * h;
     float U[3][3], q[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float is = 1.0f / s[k];
+        const float is = rcp_(s[k]);
 #pragma unroll
         for (int r = 0; r < 3; ++r) U[r][k] = (Fh[r * 3 + 0] * V[0][k] + Fh[r * 3 + 1] * V[1][k] + Fh[r * 3 + 2] * V[2][k]) * is;
         q[k] = 2.0f * mu * (sc[k] - 1.0f) * sc[k] + la * (J - 1.0f) * J;
@@ -708,9 +709,9 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
 
 __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
     if (sc.use_E) {
-        const float E = p.E[i], nu = sc.nu;
-        mu0 = E / (2.0f * (1.0f + nu));
-        la0 = E * nu / ((1.0f + nu) * (1.0f - 2.0f * nu));
+        const float E = p.E[i];
+        mu0 = E * sc.mu_per_E;
+        la0 = E * sc.la_per_E;
     } else {
         mu0 = sc.mu0;
         la0 = sc.lambda0;
@@ -724,7 +725,7 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 // its sorted position.
 template <bool TRACK>
 __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
-    __shared__ float vu[kVelCh * 512];
+    __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
     const uint32_t nh = a.cnt->n_home;
     for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
         const uint32_t slot = a.g.home_list[blk];
@@ -739,7 +740,7 @@ __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
             const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
             const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
                       Z = (tgt >> 2) * 4 + (nn >> 4);
-            vu[c * 512 + Z * 64 + Y * 8 + X] = a.g.vu[((size_t)ts * kVelCh + c) * 64 + nn];
+            vu[(Z * 64 + Y * 8 + X) * 8 + c + (c >= 3 ? 1 : 0)] = a.g.vu[((size_t)ts * kVelCh + c) * 64 + nn];
         }
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
@@ -777,8 +778,9 @@ __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
                     float S0 = 0.f, S1 = 0.f, S2 = 0.f, X0 = 0.f, X1 = 0.f, X2 = 0.f, U0 = 0.f, U1 = 0.f, U2 = 0.f;
 #pragma unroll
                     for (int aa = 0; aa < 4; ++aa) {
-                        const float gv0 = vu[0 * 512 + rowb + aa], gv1 = vu[1 * 512 + rowb + aa], gv2 = vu[2 * 512 + rowb + aa];
-                        const float gu0 = vu[3 * 512 + rowb + aa], gu1 = vu[4 * 512 + rowb + aa], gu2 = vu[5 * 512 + rowb + aa];
+                        const float4 gv = *reinterpret_cast<const float4*>(&vu[(rowb + aa) * 8]);
+                        const float4 gu = *reinterpret_cast<const float4*>(&vu[(rowb + aa) * 8 + 4]);
+                        const float gv0 = gv.x, gv1 = gv.y, gv2 = gv.z, gu0 = gu.x, gu1 = gu.y, gu2 = gu.z;
                         S0 += wx[aa] * gv0;
                         S1 += wx[aa] * gv1;
                         S2 += wx[aa] * gv2;

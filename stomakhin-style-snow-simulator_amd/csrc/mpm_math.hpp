@@ -9,6 +9,15 @@
 
 namespace snowmpm {
 
+// Hardware reciprocal / square root / reciprocal square root (about 1 ulp,
+// no IEEE division sequence) for FP32; exact operations for FP64.
+__device__ __forceinline__ float rcp_(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float rsq_(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
+__device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
+__device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
+
 // Cubic B-spline weights of the 4 stencil nodes c-1, c, c+1, c+2 for a
 // particle at fractional offset f in [0,1) inside cell c, and their
 // derivatives with respect to the particle coordinate (grid units).  These
@@ -60,19 +69,19 @@ __device__ __forceinline__ T collider_phi(const ColliderDev& c, T x0, T x1, T x2
                 n[a] = q[a] > 0 ? q[a] * (d[a] >= 0 ? (T)1 : (T)-1) : (T)0;
                 l += n[a] * n[a];
             }
-            l = (T)1 / sqrt(l);
+            l = rsq_(l);
 #pragma unroll
             for (int a = 0; a < 3; ++a) n[a] *= l;
         } else {
 #pragma unroll
             for (int a = 0; a < 3; ++a) n[a] = (a == amax) ? (d[a] >= 0 ? (T)1 : (T)-1) : (T)0;
         }
-        return sqrt(outside) + (qmax < 0 ? qmax : (T)0);
+        return sqrt_(outside) + (qmax < 0 ? qmax : (T)0);
     }
     T d0 = x0 - (T)c.point[0], d1 = x1 - (T)c.point[1], d2 = x2 - (T)c.point[2];
-    T l = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    T l = sqrt_(d0 * d0 + d1 * d1 + d2 * d2);
     if (l > 0) {
-        T il = (T)1 / l;
+        T il = rcp_(l);
         n[0] = d0 * il;
         n[1] = d1 * il;
         n[2] = d2 * il;
@@ -97,12 +106,12 @@ __device__ __forceinline__ void collide(const SceneDev& sc, T x0, T x1, T x2, T&
             T vn = r0 * n[0] + r1 * n[1] + r2 * n[2];
             if (vn < 0) {
                 T t0 = r0 - vn * n[0], t1 = r1 - vn * n[1], t2 = r2 - vn * n[2];
-                T tn = sqrt(t0 * t0 + t1 * t1 + t2 * t2);
+                T tn = sqrt_(t0 * t0 + t1 * t1 + t2 * t2);
                 T mu = (T)c.friction;
                 if (tn <= -1 * mu * vn) {
                     r0 = r1 = r2 = 0;
                 } else {
-                    T s = mu * vn / tn;
+                    T s = mu * vn * rcp_(tn);
                     r0 = t0 + s * t0;
                     r1 = t1 + s * t1;
                     r2 = t2 + s * t2;
@@ -128,12 +137,13 @@ __device__ __forceinline__ void jacobi_rot(T (&a)[3][3], T (&v)[3][3]) {
     constexpr int R = 3 - P - Q;
     const T apq = a[P][Q];
     if (apq == (T)0) return;
-    const T theta = (a[Q][Q] - a[P][P]) / (2 * apq);
+    const T theta = (a[Q][Q] - a[P][P]) * (T)0.5 * rcp_(apq);
     const T at = fabs(theta);
-    T t = (T)1 / (at + sqrt(at * at + 1));
-    if (!(at < (T)1e18)) t = (T)0.5 / at;
+    // t = 1/(|theta| + sqrt(theta^2+1)); for huge |theta| the square
+    // overflows to inf and t becomes 0, i.e. no rotation, as it should.
+    T t = rcp_(at + sqrt_(at * at + 1));
     if (theta < 0) t = -t;
-    const T c = (T)1 / sqrt(t * t + 1);
+    const T c = rsq_(t * t + 1);
     const T s = t * c;
     a[P][P] -= t * apq;
     a[Q][Q] += t * apq;
