@@ -31,7 +31,7 @@ def cpu_baseline(seed):
     import mpmref  # the oracle driver, used here only as the timed CPU baseline
     import oracle as orc
 
-    scale = 0.2  # 80 x 50 x 80 = 320,000 particles of the same slab
+    scale = 0.3  # 120 x 50 x 120 = 720,000 particles of the same slab
     sc = snowmpm.generate_scene(2, seed=seed, scale=scale)
     n = sc["x"].shape[0]
     threads = os.cpu_count() or 1
@@ -43,13 +43,32 @@ def cpu_baseline(seed):
     cs = mpmref.orc_colliders(sc["colliders"])
     op = mpmref.orc_params(prm)
     st.run(cs, op, 1, nthreads=threads)  # warm-up step
-    steps = 3
+    steps = 10
     t0 = time.perf_counter()
     st.run(cs, op, steps, nthreads=threads)
     dt = time.perf_counter() - t0
     return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"slab scale {scale}: {n} particles x {steps} steps (after 1 warm-up), FP64 oracle "
                       f"(std::unordered_map grid, reference serial/OpenMP split), {dt:.1f} s"}
+
+
+def copy_calibration(nbytes=1 << 30):
+    """One device-to-device hipMemcpy of known size: the reference point that
+    scales rocprofv3's FETCH_SIZE / WRITE_SIZE into HBM bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    t0 = time.perf_counter()
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    dt = time.perf_counter() - t0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return {"bytes": nbytes, "GB_per_s_read_plus_write": 2 * nbytes / dt / 1e9}
 
 
 def main():
@@ -59,6 +78,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
@@ -117,6 +138,8 @@ def main():
         "status": st,
         "fp64_fallbacks": fb,
     }
+    if args.copy_calibration:
+        out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.seed)
     print(json.dumps(out))
