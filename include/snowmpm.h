@@ -134,6 +134,53 @@ int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]);
  * [bin+sort, p2g, grid, g2p(+fp64 fix-up), end]; *nsteps = steps averaged. */
 int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
 
+/* The same steps replayed from a captured hipGraph (one graph per buffer
+ * parity, re-captured only when the launch setup changes); the device-side
+ * particle/block counts make one graph valid for any set of resident scenes.
+ * Status is checked once at the end. */
+int mpm_step_graph(mpm_engine* e, int nsteps);
+
+/* Chooses the persistent grid sizes of the step kernels by timing real steps
+ * (results do not depend on the launch shape, so tuning steps are ordinary
+ * steps).  steps_per_config >= 1; chosen (4 ints, may be NULL) receives
+ * {particle grid, wave-per-block grid, workgroup-per-block grid, k_blocks grid}. */
+int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]);
+
+/* ---- serving: continuous batching of scene requests ----
+ * Each request brings its particles (host arrays, V0 computed by setup) and a
+ * step count; it is admitted once it has arrived (arrival_s after the call
+ * starts) and its footprint (particles, estimated blocks) fits, steps with all
+ * resident scenes in graph-replayed chunks, and is downloaded and retired as
+ * soon as its steps are done.  mode 0: continuous batching; mode 1: the plain
+ * per-scene GPU step, one request at a time in arrival order. */
+typedef struct {
+    double arrival_s;
+    int steps;
+    int ncolliders;
+    size_t n;
+    const double* x;
+    const double* v;
+    const double* m;
+    const double* E; /* may be NULL */
+    mpm_params prm;
+    mpm_collider colliders[4];
+    double* out_x; /* n*3, may be NULL */
+    double* out_v; /* n*3, may be NULL */
+    double t_admit_s; /* filled in */
+    double t_done_s;  /* filled in */
+} mpm_request;
+
+typedef struct {
+    double wall_s;
+    double scenes_per_s;
+    double p50_s, p99_s, mean_s, max_s; /* completion latency = t_done - arrival */
+    double particle_steps;
+    int completed;
+    int max_resident;
+} mpm_serve_stats;
+
+int mpm_serve(mpm_engine* e, mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* stats);
+
 /* ---- collider factories (collision.cpp:5-27 and the box/sphere bodies) ---- */
 mpm_collider mpm_make_plane(const double point[3], const double normal[3], double friction);
 mpm_collider mpm_make_box(const double centre[3], const double half_extent[3], double friction);

@@ -24,14 +24,6 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string& s) { g_last_error = s; }
 
-#define HIP_TRY(expr)                                                                          \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess) {                                                                \
-            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
-            return MPM_ERR_HIP;                                                                \
-        }                                                                                      \
-    } while (0)
 
 static double pow2_scale(double ref) {
     // float -> fixed point: ref maps to 2^40 (headroom 2^22 before int64 overflow)
@@ -42,7 +34,16 @@ static double pow2_scale(double ref) {
 
 Engine::~Engine() { release(); }
 
+void Engine::drop_graphs() {
+    for (auto& g : gexec_)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
 void Engine::release() {
+    drop_graphs();
     for (void* p : allocs_) (void)hipFree(p);
     allocs_.clear();
     for (auto& ev : events_) (void)hipEventDestroy(ev);
@@ -75,6 +76,7 @@ int Engine::init(const mpm_config& cfg) {
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     int cus = 256;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg_.device));
+    cus_ = cus;
     const size_t N = cfg_.particle_capacity;
     launch_.grid_p = (int)std::min<size_t>((N + 255) / 256, (size_t)cus * 16);
     launch_.grid_b = cus * 8;
@@ -530,9 +532,6 @@ int Engine::scene_sums(int sid, double out[7]) {
 using snowmpm::Engine;
 using snowmpm::set_error;
 
-struct mpm_engine {
-    Engine eng;
-};
 
 extern "C" {
 

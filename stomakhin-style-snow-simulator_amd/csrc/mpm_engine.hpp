@@ -15,6 +15,15 @@ namespace snowmpm {
 extern thread_local std::string g_last_error;
 void set_error(const std::string& s);
 
+#define HIP_TRY(expr)                                                          \
+    do {                                                                       \
+        hipError_t e_ = (expr);                                                \
+        if (e_ != hipSuccess) {                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));      \
+            return MPM_ERR_HIP;                                                \
+        }                                                                      \
+    } while (0)
+
 struct SceneHost {
     bool used = false;
     bool pending_drop = false;  // removed; particles leave at the next step
@@ -42,6 +51,9 @@ class Engine {
     }
     int setup();
     int step(int nsteps, mpm_timings* t);
+    int step_graph(int nsteps);
+    int autotune(int steps_per_config, int chosen[4]);
+    int serve(mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* st);
     int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
     int scene_status(int sid, uint32_t* status, uint64_t* fb);
     int scene_sums(int sid, double out[7]);
@@ -57,6 +69,9 @@ class Engine {
     int device_count(uint32_t* n);
     int upload_scene_table(int sid);
     int check_status();
+    int ensure_graph(int parity);
+    void drop_graphs();
+    double footprint_blocks(size_t n) const;
 
     mpm_config cfg_{};
     hipStream_t stream_ = nullptr;
@@ -77,8 +92,17 @@ class Engine {
     std::vector<SceneHost> scenes_;
     std::vector<void*> allocs_;
     std::vector<hipEvent_t> events_;
+    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
+    StepArgs gargs_[2]{};
+    LaunchCfg glaunch_[2]{};
+    int cus_ = 256;
     double kernel_ms_[5] = {0, 0, 0, 0, 0};
     int kernel_steps_ = 0;
 };
 
 }  // namespace snowmpm
+
+// The opaque handle of the C ABI.
+struct mpm_engine {
+    snowmpm::Engine eng;
+};

@@ -64,6 +64,24 @@ class Config(C.Structure):
                 ("device", C.c_int), ("pad_", C.c_int), ("kappa", C.c_double)]
 
 
+class Request(C.Structure):
+    """mpm_request: one scene request for the serving loop."""
+    _fields_ = [("arrival_s", C.c_double), ("steps", C.c_int), ("ncolliders", C.c_int), ("n", C.c_size_t),
+                ("x", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)), ("m", C.POINTER(C.c_double)),
+                ("E", C.POINTER(C.c_double)), ("prm", Params), ("colliders", Collider * 4),
+                ("out_x", C.POINTER(C.c_double)), ("out_v", C.POINTER(C.c_double)),
+                ("t_admit_s", C.c_double), ("t_done_s", C.c_double)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("scenes_per_s", C.c_double), ("p50_s", C.c_double),
+                ("p99_s", C.c_double), ("mean_s", C.c_double), ("max_s", C.c_double),
+                ("particle_steps", C.c_double), ("completed", C.c_int), ("max_resident", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _pd = C.POINTER(C.c_double)
 _ppd = C.POINTER(_pd)
 _SIGS = {
@@ -81,6 +99,9 @@ _SIGS = {
     "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
+    "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "mpm_serve": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int, C.c_int, C.POINTER(ServeStats)]),
     "mpm_make_plane": (Collider, [_pd, _pd, C.c_double]),
     "mpm_make_box": (Collider, [_pd, _pd, C.c_double]),
     "mpm_make_sphere": (Collider, [_pd, C.c_double, C.c_double]),
@@ -252,6 +273,54 @@ class Engine:
     def step(self, nsteps=1, timings=None):
         """nsteps x step() (grid.cpp:359-450) on every active scene."""
         return _check(lib.mpm_step(self.h, int(nsteps), C.byref(timings) if timings is not None else None))
+
+    def step_graph(self, nsteps=1):
+        """Same steps as step(), replayed from a captured hipGraph."""
+        return _check(lib.mpm_step_graph(self.h, int(nsteps)))
+
+    def autotune(self, steps_per_config=2):
+        chosen = (C.c_int * 4)()
+        _check(lib.mpm_autotune(self.h, int(steps_per_config), chosen))
+        return dict(grid_p=chosen[0], grid_b=chosen[1], grid_w=chosen[2], grid_k=chosen[3])
+
+    def serve(self, requests, mode=0, max_chunk=25, keep_outputs=False):
+        """Runs the request loop.  requests: dicts with arrival_s, steps and a
+        generate_scene() payload (x, v, m, E, params, colliders).  mode 0 is
+        continuous batching, mode 1 the plain per-scene step one at a time."""
+        arr = (Request * len(requests))()
+        keep = []
+        outs = []
+        for k, q in enumerate(requests):
+            r = arr[k]
+            r.arrival_s = float(q["arrival_s"])
+            r.steps = int(q["steps"])
+            x = np.ascontiguousarray(q["x"], dtype=np.float64)
+            v = np.ascontiguousarray(q["v"], dtype=np.float64)
+            m = np.ascontiguousarray(q["m"], dtype=np.float64)
+            E = None if q.get("E") is None else np.ascontiguousarray(q["E"], dtype=np.float64)
+            keep += [x, v, m, E]
+            r.n = x.shape[0]
+            r.x, r.v, r.m, r.E = _dptr(x), _dptr(v), _dptr(m), _dptr(E)
+            r.prm = q["params"]
+            cs = q["colliders"]
+            r.ncolliders = len(cs)
+            for j, c in enumerate(cs):
+                r.colliders[j] = c
+            if keep_outputs:
+                ox = np.zeros_like(x)
+                ov = np.zeros_like(v)
+                outs.append((ox, ov))
+                keep += [ox, ov]
+                r.out_x, r.out_v = _dptr(ox), _dptr(ov)
+            else:
+                outs.append(None)
+        st = ServeStats()
+        _check(lib.mpm_serve(self.h, arr, len(requests), int(mode), int(max_chunk), C.byref(st)))
+        res = st.as_dict()
+        res["latency_s"] = [arr[k].t_done_s - arr[k].arrival_s for k in range(len(requests))]
+        if keep_outputs:
+            res["outputs"] = outs
+        return res
 
     def step_rc(self, nsteps=1):
         """Like step() but returns the status code instead of raising."""

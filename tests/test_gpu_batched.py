@@ -1,0 +1,84 @@
+"""GPU tests of the batched and served forms: graph-replayed steps, the
+autotuner and the continuous-batching request loop give the same bits as the
+plain per-scene step (launch shape and batch composition never enter the
+arithmetic)."""
+import numpy as np
+import pytest
+
+import mpmref
+import snowmpm
+
+pytestmark = pytest.mark.gpu
+
+
+def _scenes(k, scale=0.08, seed=5):
+    return [snowmpm.generate_scene(3, seed=seed, index=i, scale=scale) for i in range(k)]
+
+
+def test_graph_steps_equal_plain_steps():
+    sc = snowmpm.generate_scene(1, seed=3, scale=0.12)
+    a = mpmref.run_gpu(sc, 9)
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as eng:
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        eng.step_graph(4)
+        eng.step(1)
+        eng.step_graph(4)
+        b = eng.download(sid)
+    for k in ("x", "v", "Fe", "Jp"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_autotune_keeps_results():
+    sc = snowmpm.generate_scene(0, seed=3, scale=0.3)
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as eng:
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        eng.setup()
+        chosen = eng.autotune(1)
+        assert all(v > 0 for v in chosen.values())
+        # the tuner ran ordinary steps: count them from a reference run
+        got = eng.download(sid)
+    nsteps_tuner = None
+    for n in range(2, 20):
+        ref = mpmref.run_gpu(sc, n)
+        if np.array_equal(ref["x"], got["x"]):
+            nsteps_tuner = n
+            break
+    assert nsteps_tuner is not None
+
+
+def test_served_results_equal_alone():
+    scs = _scenes(6)
+    reqs = []
+    for i, sc in enumerate(scs):
+        reqs.append(dict(sc, arrival_s=0.002 * i, steps=7 + 3 * i))
+    cap = sum(len(s["x"]) for s in scs) + 1024
+    with snowmpm.Engine(capacity=cap) as eng:
+        res = eng.serve(reqs, mode=0, max_chunk=4, keep_outputs=True)
+    assert res["completed"] == len(reqs)
+    assert res["max_resident"] >= 2
+    for q, (ox, ov) in zip(reqs, res["outputs"]):
+        alone = mpmref.run_gpu(q, q["steps"])
+        assert np.array_equal(alone["x"], ox)
+        assert np.array_equal(alone["v"], ov)
+    assert res["p99_s"] >= res["p50_s"] > 0
+
+
+def test_plain_serving_mode():
+    scs = _scenes(3)
+    reqs = [dict(sc, arrival_s=0.0, steps=5) for sc in scs]
+    with snowmpm.Engine(capacity=max(len(s["x"]) for s in scs) + 64) as eng:
+        res = eng.serve(reqs, mode=1, keep_outputs=True)
+    assert res["completed"] == 3 and res["max_resident"] == 1
+    for q, (ox, ov) in zip(reqs, res["outputs"]):
+        assert np.array_equal(mpmref.run_gpu(q, 5)["x"], ox)
+
+
+def test_admission_waits_for_capacity():
+    scs = _scenes(4)
+    n_max = max(len(s["x"]) for s in scs)
+    reqs = [dict(sc, arrival_s=0.0, steps=6) for sc in scs]
+    with snowmpm.Engine(capacity=n_max + 8) as eng:  # room for one scene at a time
+        res = eng.serve(reqs, mode=0, max_chunk=3, keep_outputs=True)
+    assert res["completed"] == 4 and res["max_resident"] == 1
+    for q, (ox, _) in zip(reqs, res["outputs"]):
+        assert np.array_equal(mpmref.run_gpu(q, 6)["x"], ox)
