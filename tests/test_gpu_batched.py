@@ -50,7 +50,7 @@ def test_served_results_equal_alone():
     scs = _scenes(6)
     reqs = []
     for i, sc in enumerate(scs):
-        reqs.append(dict(sc, arrival_s=0.002 * i, steps=7 + 3 * i))
+        reqs.append(dict(sc, arrival_s=0.0 if i < 3 else 0.001 * i, steps=7 + 3 * i))
     cap = sum(len(s["x"]) for s in scs) + 1024
     with snowmpm.Engine(capacity=cap) as eng:
         res = eng.serve(reqs, mode=0, max_chunk=4, keep_outputs=True)
@@ -77,8 +77,10 @@ def test_admission_waits_for_capacity():
     scs = _scenes(4)
     n_max = max(len(s["x"]) for s in scs)
     reqs = [dict(sc, arrival_s=0.0, steps=6) for sc in scs]
-    with snowmpm.Engine(capacity=n_max + 8) as eng:  # room for one scene at a time
+    cap = n_max + 8
+    assert sum(len(s["x"]) for s in scs) > cap  # not all four fit at once
+    with snowmpm.Engine(capacity=cap) as eng:
         res = eng.serve(reqs, mode=0, max_chunk=3, keep_outputs=True)
-    assert res["completed"] == 4 and res["max_resident"] == 1
+    assert res["completed"] == 4 and 1 <= res["max_resident"] < 4
     for q, (ox, _) in zip(reqs, res["outputs"]):
         assert np.array_equal(mpmref.run_gpu(q, 6)["x"], ox)
