@@ -9,7 +9,11 @@ average launch time measured with HIP events on the engine's stream.
 The CPU baseline is the FP64 oracle (the reference algorithm restated, same
 serial/OpenMP split) timed on a bounded sample of the same slab.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The line also carries the batched form (256 uneven scenes, graph-replayed,
+autotuned) and the served form (Poisson arrivals at 10-200 requests/s,
+continuous batching) with p50/p99 latency, beside the plain per-scene GPU step.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving]
 """
 import argparse
 import json
@@ -71,6 +75,82 @@ def copy_calibration(nbytes=1 << 30):
     return {"bytes": nbytes, "GB_per_s_read_plus_write": 2 * nbytes / dt / 1e9}
 
 
+def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(10, 25, 50, 100, 200),
+                       requests_per_rate=64):
+    """Batched form (256 uneven scenes in one engine, graph-replayed step,
+    autotuned launch shapes) and served form (Poisson arrivals, continuous
+    batching) against the plain per-scene GPU step one scene at a time.
+    All forms give bitwise-identical per-scene results (tests/test_gpu_batched.py)."""
+    import numpy as np
+    import snowmpm
+
+    scenes = [snowmpm.generate_scene(3, seed=seed, index=i) for i in range(n_scenes)]
+    ns = np.array([len(s["x"]) for s in scenes])
+    total = int(ns.sum())
+    out = {"scenes": n_scenes, "particles_total": total, "particles_min": int(ns.min()),
+           "particles_max": int(ns.max()), "steps_per_scene": steps}
+    # batched: all scenes resident, one launch per stage per step
+    eng = snowmpm.Engine(capacity=total + 4096)
+    for s in scenes:
+        eng.add_scene(s["params"], s["colliders"], s["x"], s["v"], s["m"])
+    eng.setup()
+    out["autotune"] = eng.autotune(2)
+    eng.step_graph(5)
+    t0 = time.perf_counter()
+    eng.step_graph(steps)
+    wall = time.perf_counter() - t0
+    eng.close()
+    out["batched_scenes_per_s"] = n_scenes / wall
+    out["batched_ms_per_step"] = wall * 1e3 / steps
+    out["batched_particle_steps_per_s"] = total * steps / wall
+    # plain: one scene at a time, plain launches (sampled)
+    idx = np.linspace(0, n_scenes - 1, plain_sample).astype(int)
+    eng = snowmpm.Engine(capacity=int(ns.max()) + 4096)
+    t_plain = 0.0
+    for i in idx:
+        s = scenes[i]
+        t0 = time.perf_counter()
+        sid = eng.add_scene(s["params"], s["colliders"], s["x"], s["v"], s["m"])
+        eng.step(steps)
+        t_plain += time.perf_counter() - t0
+        eng.clear()
+    eng.close()
+    plain_per_scene = t_plain / len(idx) * (ns.mean() / ns[idx].mean())
+    out["plain_scenes_per_s"] = 1.0 / plain_per_scene
+    out["plain_sample"] = f"{len(idx)} of the {n_scenes} scenes, {steps} steps each, scaled by mean size"
+    out["batched_vs_plain"] = out["batched_scenes_per_s"] / out["plain_scenes_per_s"]
+
+    # served: Poisson arrivals, steps ~ U{100..2000}
+    rng = np.random.default_rng(seed)
+    served = []
+    cap = 16_000_000
+    eng = snowmpm.Engine(capacity=cap)
+    for lam in rates:
+        gaps = rng.exponential(1.0 / lam, size=requests_per_rate)
+        arr = np.cumsum(gaps) - gaps[0]
+        reqs = []
+        for k in range(requests_per_rate):
+            s = scenes[int(rng.integers(0, n_scenes))]
+            reqs.append(dict(s, arrival_s=float(arr[k]), steps=int(rng.integers(100, 2001))))
+        r = eng.serve(reqs, mode=0, max_chunk=25)
+        served.append({"rate_per_s": lam, "requests": requests_per_rate, "scenes_per_s": r["scenes_per_s"],
+                       "p50_s": r["p50_s"], "p99_s": r["p99_s"], "mean_s": r["mean_s"],
+                       "max_resident": r["max_resident"], "wall_s": r["wall_s"]})
+    eng.close()
+    # plain serving at saturation (all requests queued at t=0), sampled
+    eng = snowmpm.Engine(capacity=int(ns.max()) + 4096)
+    reqs = []
+    for k in range(8):
+        s = scenes[int(rng.integers(0, n_scenes))]
+        reqs.append(dict(s, arrival_s=0.0, steps=int(rng.integers(100, 2001))))
+    r = eng.serve(reqs, mode=1)
+    eng.close()
+    out["served"] = served
+    out["served_plain_saturated"] = {"requests": 8, "scenes_per_s": r["scenes_per_s"], "p50_s": r["p50_s"],
+                                     "p99_s": r["p99_s"]}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +158,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true", help="skip the batched / served measurements")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -138,6 +219,8 @@ def main():
         "status": st,
         "fp64_fallbacks": fb,
     }
+    if not args.no_serving:
+        out["batched"] = batched_and_served(args.seed)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
