@@ -126,40 +126,46 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
                 }
             }
         }
-        // one hash insert per distinct block key in the wave
-        uint32_t slot = kInvalid;
+        // Leaders: the lowest lane of each distinct block key (ballots only),
+        // then every leader inserts in the same instruction.
+        int kleader = (int)lane;
         uint64_t pending = __ballot(want);
         while (pending) {
             const int leader = __ffsll((unsigned long long)pending) - 1;
             const uint32_t klo = __shfl((uint32_t)key, leader), khi = __shfl((uint32_t)(key >> 32), leader);
-            const uint64_t lkey = ((uint64_t)khi << 32) | klo;
-            const bool match = want && key == lkey;
-            uint32_t s_ = 0;
-            if ((int)lane == leader) {
-                bool ins;
-                s_ = hash_insert(a.g, lkey, ins);
-                if (s_ == kInvalid) flag(a.st, sid, kStatusHashFull);
-            }
-            s_ = __shfl(s_, leader);
-            if (match) slot = s_;
+            const bool match = want && key == (((uint64_t)khi << 32) | klo);
+            if (match) kleader = leader;
             pending &= ~__ballot(match);
         }
-        // one counter update per distinct cell in the wave
+        uint32_t s_ = kInvalid;
+        if (want && kleader == (int)lane) {
+            bool ins;
+            s_ = hash_insert(a.g, key, ins);
+            if (s_ == kInvalid) flag(a.st, sid, kStatusHashFull);
+        }
+        const uint32_t slot_b = __shfl(s_, kleader);
+        const uint32_t slot = want ? slot_b : kInvalid;
+        // Same for cells: one counter bump per distinct cell, all at once.
         const bool have = slot != kInvalid;
         const uint32_t ck = have ? (slot << 6) | l : kInvalid;
-        uint32_t r = 0;
+        int cleader = (int)lane;
+        uint64_t cmask = 0;
         uint64_t pend2 = __ballot(have);
         while (pend2) {
             const int leader = __ffsll((unsigned long long)pend2) - 1;
             const uint32_t lck = __shfl(ck, leader);
             const bool match = have && ck == lck;
             const uint64_t mm = __ballot(match);
-            uint32_t basecnt = 0;
-            if ((int)lane == leader) basecnt = atomicAdd(&a.g.cellcnt[lck], (uint32_t)__popcll(mm));
-            basecnt = __shfl(basecnt, leader);
-            if (match) r = basecnt + (uint32_t)__popcll(mm & lt_mask);
+            if (match) {
+                cleader = leader;
+                cmask = mm;
+            }
             pend2 &= ~mm;
         }
+        uint32_t basecnt = 0;
+        if (have && cleader == (int)lane) basecnt = atomicAdd(&a.g.cellcnt[ck], (uint32_t)__popcll(cmask));
+        basecnt = __shfl(basecnt, cleader);
+        const uint32_t r = have ? basecnt + (uint32_t)__popcll(cmask & lt_mask) : 0u;
         if (i < n) {
             a.bin.pkey[i] = ck;
             a.bin.prank[i] = r;
@@ -285,24 +291,58 @@ __global__ __launch_bounds__(256) void k_scatter(StepArgs a) {
 
 // Insertion sort of each cell's segment by particle id: the P2G summation
 // order inside a cell then depends only on the particles, not on the order
-// in which atomics handed out ranks.
+// in which atomics handed out ranks.  One wave per block; the block's range is
+// sorted in LDS (segments are almost sorted already: the buffer keeps the
+// previous step's order) and written back.
+constexpr int kSortCap = 1024;
 __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
+    __shared__ uint32_t kp[4][kSortCap];
+    __shared__ uint32_t ki[4][kSortCap];
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t nh = a.cnt->n_home;
     for (uint32_t b = blockIdx.x * 4 + wid; b < nh; b += gridDim.x * 4) {
         const uint32_t slot = a.g.home_list[b];
+        const uint32_t off = a.g.blkoff[slot], nb = a.g.blkcnt[slot];
         const uint32_t cnt = a.g.cellcnt[(size_t)slot * 64 + lane];
-        const uint32_t s0 = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + lane];
-        for (uint32_t k = 1; k < cnt; ++k) {
-            const uint32_t kp = a.bin.spid[s0 + k], ki = a.bin.sidx[s0 + k];
-            uint32_t j = k;
-            while (j > 0 && a.bin.spid[s0 + j - 1] > kp) {
-                a.bin.spid[s0 + j] = a.bin.spid[s0 + j - 1];
-                a.bin.sidx[s0 + j] = a.bin.sidx[s0 + j - 1];
-                --j;
+        const uint32_t c0 = a.g.celloff[(size_t)slot * 64 + lane];
+        if (nb <= (uint32_t)kSortCap) {
+            uint32_t* P = kp[wid];
+            uint32_t* I = ki[wid];
+            for (uint32_t t = lane; t < nb; t += 64) {
+                P[t] = a.bin.spid[off + t];
+                I[t] = a.bin.sidx[off + t];
             }
-            a.bin.spid[s0 + j] = kp;
-            a.bin.sidx[s0 + j] = ki;
+            __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+            for (uint32_t k = 1; k < cnt; ++k) {
+                const uint32_t kpk = P[c0 + k], kik = I[c0 + k];
+                uint32_t j = k;
+                while (j > 0 && P[c0 + j - 1] > kpk) {
+                    P[c0 + j] = P[c0 + j - 1];
+                    I[c0 + j] = I[c0 + j - 1];
+                    --j;
+                }
+                P[c0 + j] = kpk;
+                I[c0 + j] = kik;
+            }
+            __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+            for (uint32_t t = lane; t < nb; t += 64) {
+                a.bin.spid[off + t] = P[t];
+                a.bin.sidx[off + t] = I[t];
+            }
+            __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+        } else {
+            const uint32_t s0 = off + c0;
+            for (uint32_t k = 1; k < cnt; ++k) {
+                const uint32_t kpk = a.bin.spid[s0 + k], kik = a.bin.sidx[s0 + k];
+                uint32_t j = k;
+                while (j > 0 && a.bin.spid[s0 + j - 1] > kpk) {
+                    a.bin.spid[s0 + j] = a.bin.spid[s0 + j - 1];
+                    a.bin.sidx[s0 + j] = a.bin.sidx[s0 + j - 1];
+                    --j;
+                }
+                a.bin.spid[s0 + j] = kpk;
+                a.bin.sidx[s0 + j] = kik;
+            }
         }
     }
 }
@@ -323,7 +363,7 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // and added to the owning blocks with integer atomics (order independent).
 constexpr int kP2GNS = 12;                 // floats per node in a deposit region (7 used; 12 spreads banks)
 constexpr int kP2GW = 4 * 49 * kP2GNS;     // one wave region: 4 z-planes x 7 x 7 nodes
-constexpr int kP2GChunk = 512;             // particles staged per pass
+constexpr int kP2GChunk = 512;             // particles staged per pass (>= a typical block)
 constexpr int kP2GStride = kP2GChunk + kP2GChunk / 32;  // per-field stride after the k + k/32 swizzle
 constexpr int kP2GFields = 13;             // fx fy fz m mv0 mv1 mv2 s00 s11 s22 s01 s02 s12
 
@@ -707,6 +747,71 @@ __device__ __forceinline__ bool plastic_update(const float (&Fe)[9], const float
     return false;
 }
 
+// Elastic trial step.  If the Gershgorin discs of B = Fh Fh^T lie inside
+// [(1-theta_c)^2, (1+theta_s)^2] (and inside [0.9, 1.1]), no singular value of
+// Fh leaves the clamp interval, so the plasticity update is the identity on
+// F_E and J_P (F_E' = Fh, as U Sigma V^T = Fh), and the stress needs no
+// factorisation: P(F) F^T = 2 mu (B - sqrt(B)) + lambda (J-1) J I with
+// B - sqrt(B) = E/2 + E^2/8 - E^3/16 + 5E^4/128 - 7E^5/256 + O(E^6), E = B - I
+// (|E| <= 0.1 bounds the truncation below 2e-8 relative).  E is symmetric and
+// every Horner factor is a polynomial in E, so all products are symmetric.
+struct Sym {
+    float a00, a11, a22, a01, a02, a12;
+};
+__device__ __forceinline__ Sym sym_mul(const Sym& A, const Sym& B) {
+    Sym C;
+    C.a00 = A.a00 * B.a00 + A.a01 * B.a01 + A.a02 * B.a02;
+    C.a11 = A.a01 * B.a01 + A.a11 * B.a11 + A.a12 * B.a12;
+    C.a22 = A.a02 * B.a02 + A.a12 * B.a12 + A.a22 * B.a22;
+    C.a01 = A.a00 * B.a01 + A.a01 * B.a11 + A.a02 * B.a12;
+    C.a02 = A.a00 * B.a02 + A.a01 * B.a12 + A.a02 * B.a22;
+    C.a12 = A.a01 * B.a02 + A.a11 * B.a12 + A.a12 * B.a22;
+    return C;
+}
+
+__device__ __forceinline__ bool elastic_trial(const float Fh[9], float tc, float ts, Sym& E) {
+    E.a00 = Fh[0] * Fh[0] + Fh[1] * Fh[1] + Fh[2] * Fh[2] - 1.0f;
+    E.a11 = Fh[3] * Fh[3] + Fh[4] * Fh[4] + Fh[5] * Fh[5] - 1.0f;
+    E.a22 = Fh[6] * Fh[6] + Fh[7] * Fh[7] + Fh[8] * Fh[8] - 1.0f;
+    E.a01 = Fh[0] * Fh[3] + Fh[1] * Fh[4] + Fh[2] * Fh[5];
+    E.a02 = Fh[0] * Fh[6] + Fh[1] * Fh[7] + Fh[2] * Fh[8];
+    E.a12 = Fh[3] * Fh[6] + Fh[4] * Fh[7] + Fh[5] * Fh[8];
+    const float r0 = fabsf(E.a01) + fabsf(E.a02), r1 = fabsf(E.a01) + fabsf(E.a12), r2 = fabsf(E.a02) + fabsf(E.a12);
+    const float lo = fminf(E.a00 - r0, fminf(E.a11 - r1, E.a22 - r2));
+    const float hi = fmaxf(E.a00 + r0, fmaxf(E.a11 + r1, E.a22 + r2));
+    const float c = 1.0f - tc, t = 1.0f + ts;
+    return lo >= fmaxf(c * c - 1.0f, -0.1f) && hi <= fminf(t * t - 1.0f, 0.1f) && det3<float>(Fh) > 0.0f;
+}
+
+__device__ __forceinline__ void elastic_stress(const Sym& E, float J, float mu, float la, float V0, float S[6]) {
+    Sym T;
+    T.a00 = T.a11 = T.a22 = 5.0f / 128.0f;
+    T.a01 = T.a02 = T.a12 = 0.0f;
+    const float c5 = -7.0f / 256.0f;
+    T.a00 += c5 * E.a00;
+    T.a11 += c5 * E.a11;
+    T.a22 += c5 * E.a22;
+    T.a01 = c5 * E.a01;
+    T.a02 = c5 * E.a02;
+    T.a12 = c5 * E.a12;
+    const float cs[3] = {-1.0f / 16.0f, 1.0f / 8.0f, 0.5f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        T = sym_mul(E, T);
+        T.a00 += cs[k];
+        T.a11 += cs[k];
+        T.a22 += cs[k];
+    }
+    const Sym f = sym_mul(E, T);  // B - sqrt(B)
+    const float two_mu = 2.0f * mu, iso = la * (J - 1.0f) * J, nv = -V0;
+    S[0] = nv * (two_mu * f.a00 + iso);
+    S[1] = nv * (two_mu * f.a11 + iso);
+    S[2] = nv * (two_mu * f.a22 + iso);
+    S[3] = nv * (two_mu * f.a01);
+    S[4] = nv * (two_mu * f.a02);
+    S[5] = nv * (two_mu * f.a12);
+}
+
 __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
     if (sc.use_E) {
         const float E = p.E[i];
@@ -821,8 +926,29 @@ __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
 #pragma unroll
             for (int e = 0; e < 9; ++e) Fp_old[e] = TRACK ? a.cur.Fp[e][i] : 0.f;
             PlastOut po;
-            const bool fb = plastic_update<TRACK>(Fe, L, dt, Jp, Fp_old, Fp_new, true, sc.theta_c, sc.theta_s, mu0,
-                                                  la0, sc.hardening, V0, a.kappa, po, true);
+            float Fh[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    Fh[r * 3 + c] = Fe[r * 3 + c] + dt * (L[r * 3 + 0] * Fe[0 * 3 + c] + L[r * 3 + 1] * Fe[1 * 3 + c] +
+                                                          L[r * 3 + 2] * Fe[2 * 3 + c]);
+            Sym Es;
+            const bool elastic = elastic_trial(Fh, sc.theta_c, sc.theta_s, Es);
+            bool fb = false;
+            if (elastic) {
+                const float h = expf(sc.hardening * (1.0f - Jp));
+#pragma unroll
+                for (int e = 0; e < 9; ++e) {
+                    po.Fe[e] = Fh[e];
+                    Fp_new[e] = Fp_old[e];
+                }
+                po.Jp = Jp;
+                elastic_stress(Es, det3<float>(Fh), mu0 * h, la0 * h, V0, po.S);
+            } else {
+                fb = plastic_update<TRACK>(Fe, L, dt, Jp, Fp_old, Fp_new, true, sc.theta_c, sc.theta_s, mu0, la0,
+                                           sc.hardening, V0, a.kappa, po, true);
+            }
             if (fb) {
                 // Redone in FP64 by k_fp64_fixup; keep finite placeholders meanwhile.
                 const uint32_t q = atomicAdd(&a.cnt->n_fallback, 1u);
