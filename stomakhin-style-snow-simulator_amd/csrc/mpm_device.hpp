@@ -20,13 +20,14 @@ constexpr int kCoordBits = 17;              // per axis, biased
 constexpr int kCoordBias = 1 << 16;         // block coordinate range +-65534
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kBlockNodes = 64;             // 4x4x4 nodes per block
-constexpr int kGridCh = 7;                  // int64 accumulators: m, p(3), f(3)
+constexpr int kGridCh = 7;                  // P2G channels: m, p(3), f(3)
+constexpr int kRegion = 343;                // 7x7x7 nodes a home block's particles reach
 constexpr int kVelCh = 6;                   // FP32 outputs: v(3), u(3) (u = PIC/FLIP blend)
 constexpr int kStressCh = 6;                // symmetric force matrix -V0 P F^T
 
 // Status bits (per scene, OR-ed by kernels).
 constexpr uint32_t kStatusNaN = 0x1;
-constexpr uint32_t kStatusOverflow = 0x2;
+constexpr uint32_t kStatusOverflow = 0x2;   // block list capacity exceeded
 constexpr uint32_t kStatusHashFull = 0x4;
 constexpr uint32_t kStatusRange = 0x8;
 
@@ -46,8 +47,6 @@ struct SceneDev {
     float mu0, lambda0, nu;
     float mu_per_E, la_per_E;                // mu0 = E mu_per_E, lambda0 = E la_per_E
     float hardening, theta_c, theta_s, alpha;
-    float scale_m, scale_p, scale_f;         // float -> fixed point (powers of two)
-    double inv_scale_m, inv_scale_p, inv_scale_f;
     int use_E;                               // per-particle Young's modulus present
     int ncoll;
     int flags;                               // kSceneActive | kSceneSetup
@@ -92,7 +91,10 @@ struct GridDev {
     uint32_t* blkoff;      // [H] first sorted position of the block
     uint32_t* blkcnt;      // [H] particles in the block
     uint32_t* nb;          // [H*8] slots of the block + {0,1}^3 neighbours
-    unsigned long long* acc;  // [H*7*64] fixed-point accumulators (zero between passes)
+    float* part;           // [max_home*7*343] each home block's P2G partial sums
+    uint32_t* rev;         // [H*8] home-list index of block - {0,1}^3, or kInvalid
+    uint32_t max_home;     // capacity of part (home blocks per pass)
+    uint32_t pad_;
     float* vu;             // [H*6*64] grid velocity and PIC/FLIP blend
     uint32_t* home_list;   // [H]
     uint32_t* all_list;    // [H]
@@ -113,7 +115,6 @@ struct FallbackDev {
 struct StatusDev {
     uint32_t* scene_status;          // [kMaxScenes]
     unsigned long long* fp64_count;  // [kMaxScenes] FP64 SVD fallbacks
-    float* vmax;                     // [kMaxScenes] scratch for scale reductions
 };
 
 __host__ __device__ inline uint64_t pack_key(uint32_t sid, int bx, int by, int bz) {

@@ -24,14 +24,6 @@ thread_local std::string g_last_error;
 
 void set_error(const std::string& s) { g_last_error = s; }
 
-
-static double pow2_scale(double ref) {
-    // float -> fixed point: ref maps to 2^40 (headroom 2^22 before int64 overflow)
-    if (!(ref > 0.0) || !std::isfinite(ref)) ref = 1.0;
-    int e = (int)std::ceil(std::log2(ref));
-    return std::ldexp(1.0, 40 - e);
-}
-
 Engine::~Engine() { release(); }
 
 void Engine::drop_graphs() {
@@ -134,13 +126,17 @@ int Engine::init(const mpm_config& cfg) {
     if ((rc = dalloc((void**)&grid_.blkoff, (size_t)H_ * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.blkcnt, (size_t)H_ * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.nb, (size_t)H_ * 8 * 4))) return rc;
-    if ((rc = dalloc((void**)&grid_.acc, (size_t)H_ * kGridCh * 64 * 8))) return rc;
+    // P2G partials: one 7x7x7x7 float region per home block; a home block
+    // holds at least one particle, so N bounds the count as well as H / 2.
+    grid_.max_home = (uint32_t)std::min<size_t>(N + 64, H_ / 2);
+    if ((rc = dalloc((void**)&grid_.part, (size_t)grid_.max_home * kGridCh * kRegion * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.rev, (size_t)H_ * 8 * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.vu, (size_t)H_ * kVelCh * 64 * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.home_list, (size_t)H_ * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.all_list, (size_t)H_ * 4))) return rc;
     HIP_TRY(hipMemsetAsync(grid_.keys, 0xFF, (size_t)H_ * 8, stream_));
     HIP_TRY(hipMemsetAsync(grid_.cellcnt, 0, (size_t)H_ * 64 * 4, stream_));
-    HIP_TRY(hipMemsetAsync(grid_.acc, 0, (size_t)H_ * kGridCh * 64 * 8, stream_));
+    HIP_TRY(hipMemsetAsync(grid_.rev, 0xFF, (size_t)H_ * 8 * 4, stream_));
 
     if ((rc = dalloc((void**)&cnt_, sizeof(Counters)))) return rc;
     HIP_TRY(hipMemsetAsync(cnt_, 0, sizeof(Counters), stream_));
@@ -148,7 +144,6 @@ int Engine::init(const mpm_config& cfg) {
     HIP_TRY(hipMemsetAsync(d_scenes_, 0, sizeof(SceneDev) * kMaxScenes, stream_));
     if ((rc = dalloc((void**)&st_.scene_status, 4 * kMaxScenes))) return rc;
     if ((rc = dalloc((void**)&st_.fp64_count, 8 * kMaxScenes))) return rc;
-    if ((rc = dalloc((void**)&st_.vmax, 4 * kMaxScenes))) return rc;
     HIP_TRY(hipMemsetAsync(st_.scene_status, 0, 4 * kMaxScenes, stream_));
     HIP_TRY(hipMemsetAsync(st_.fp64_count, 0, 8 * kMaxScenes, stream_));
     if ((rc = dalloc((void**)&d_sums_, sizeof(double) * 7 * 1024))) return rc;
@@ -227,8 +222,6 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     const int nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1;
     std::vector<float> stage((size_t)nf * n);
     std::vector<uint32_t> pid(n), sidv(n, (uint32_t)sid);
-    double mmax = 0, vmax2 = 0, fref = 0;
-    const double nu = prm.nu;
     for (size_t i = 0; i < n; ++i) {
         size_t o = 0;
         for (int e = 0; e < 3; ++e, o += n) stage[o + i] = (float)x[i * 3 + e];
@@ -249,16 +242,6 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
         o += n;
         stage[o + i] = E ? (float)E[i] : 0.0f;
         pid[i] = (uint32_t)i;
-        mmax = std::max(mmax, std::fabs(m[i]));
-        vmax2 = std::max(vmax2, v[i * 3] * v[i * 3] + v[i * 3 + 1] * v[i * 3 + 1] + v[i * 3 + 2] * v[i * 3 + 2]);
-        if (V0) {
-            double mu = prm.mu0, la = prm.lambda0;
-            if (E) {
-                mu = E[i] / (2 * (1 + nu));
-                la = E[i] * nu / ((1 + nu) * (1 - 2 * nu));
-            }
-            fref = std::max(fref, std::fabs(V0[i]) * (mu + la));
-        }
     }
     const PBuf& p = buf_[cur_];
     {
@@ -295,14 +278,6 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     sd.theta_c = (float)prm.max_compression;
     sd.theta_s = (float)prm.max_stretch;
     sd.alpha = (float)prm.alpha;
-    const double sm = pow2_scale(mmax), sp = pow2_scale(mmax * std::max(1.0, std::sqrt(vmax2)));
-    const double sf = pow2_scale(V0 ? fref / prm.dx : 1.0);
-    sd.scale_m = (float)sm;
-    sd.scale_p = (float)sp;
-    sd.scale_f = (float)sf;
-    sd.inv_scale_m = 1.0 / sm;
-    sd.inv_scale_p = 1.0 / sp;
-    sd.inv_scale_f = 1.0 / sf;
     sd.use_E = E ? 1 : 0;
     sd.ncoll = ncoll;
     for (int k = 0; k < ncoll; ++k) {
@@ -373,18 +348,9 @@ int Engine::setup() {
     StepArgs a = args(kSceneSetup);
     launch_setup(a, launch_, stream_);
     HIP_TRY(hipGetLastError());
-    // Force scale from the new volumes: max V0 (mu0 + lambda0) per scene.
-    HIP_TRY(hipMemsetAsync(st_.vmax, 0, 4 * kMaxScenes, stream_));
-    launch_scene_max(a, launch_, 0, stream_);
-    std::vector<float> vmax(scenes_.size());
-    HIP_TRY(hipMemcpyAsync(vmax.data(), st_.vmax, 4 * scenes_.size(), hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipStreamSynchronize(stream_));
     for (size_t s = 0; s < scenes_.size(); ++s) {
         if (!scenes_[s].used || !(h_scenes_[s].flags & kSceneSetup)) continue;
         SceneDev& sd = h_scenes_[s];
-        const double sf = pow2_scale((double)vmax[s] / (double)sd.dx);
-        sd.scale_f = (float)sf;
-        sd.inv_scale_f = 1.0 / sf;
         sd.flags = kSceneActive;
         int rc = upload_scene_table((int)s);
         if (rc) return rc;

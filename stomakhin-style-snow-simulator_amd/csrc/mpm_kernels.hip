@@ -8,8 +8,10 @@
 //   k_scatter   counting-sort scatter of particle indices by (block, cell)
 //   k_cellsort  order each cell by particle id (makes P2G order canonical)
 //   k_p2g       P2G of mass, momentum and stress force (grid.cpp:119-129,
-//               162-182), LDS staged, 64-bit fixed-point global atomics
-//   k_grid      grid velocity + grid collisions (grid.cpp:195-243) fused
+//               162-182), LDS staged; each home block writes the partial
+//               sums of the 7^3 nodes its particles reach (no atomics)
+//   k_grid      sums the <= 8 partials of each node in a fixed order, then
+//               grid velocity + grid collisions (grid.cpp:195-243) fused
 //   k_g2p       stress_update + G2P + particle collisions + advection
 //               (grid.cpp:245-338) fused, also producing the next step's
 //               force matrix -V0 P F_E^T so P2G needs no SVD
@@ -254,6 +256,9 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                     }
                 }
                 a.g.nb[(size_t)slot * 8 + lane] = ns;
+                // The block H + t learns that home block H (home-list index b)
+                // holds partial sums for it; only H writes entry t of H + t.
+                if (b < a.g.max_home) a.g.rev[(size_t)ns * 8 + lane] = b;
             }
         }
         if (lane == 0) tot[wid] = total;
@@ -480,43 +485,39 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
             }
         __syncthreads();
 
-        // Flush the 7x7x7 node region into the 8 owning blocks.
-        for (int t = threadIdx.x; t < 8 * kGridCh * 64; t += 256) {
-            const int tgt = t / (kGridCh * 64);
-            const int rem = t - tgt * (kGridCh * 64);
-            const int c = rem >> 6, nn = rem & 63;
-            const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
-                      Z = (tgt >> 2) * 4 + (nn >> 4);
-            if (X > 6 || Y > 6 || Z > 6) continue;
-            float val = 0.0f;
+        // Partial sums of the 7x7x7 node region: the four wave regions added in
+        // fixed order, stored with plain coalesced writes.
+        if (blk < a.g.max_home) {
+            float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
+            for (int t = threadIdx.x; t < kGridCh * kRegion; t += 256) {
+                const int c = t / kRegion, node = t - c * kRegion;
+                const int Z = node / 49, Y = (node / 7) % 7, X = node % 7;
+                float val = 0.0f;
 #pragma unroll
-            for (int ww = 0; ww < 4; ++ww) {
-                const int zl = Z - ww;
-                if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
-            }
-            if (val != 0.0f) {
-                const float scale = c == 0 ? sc.scale_m : (c < 4 ? sc.scale_p : sc.scale_f);
-                double q = (double)val * (double)scale;
-                if (!(fabs(q) < 4.0e18)) {
-                    flag(a.st, sid, isfinite(q) ? kStatusOverflow : kStatusNaN);
-                    q = 0.0;
+                for (int ww = 0; ww < 4; ++ww) {
+                    const int zl = Z - ww;
+                    if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
                 }
-                const long long qi = __double2ll_rn(q);
-                const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
-                atomicAdd(&a.g.acc[((size_t)ts * kGridCh + c) * 64 + nn], (unsigned long long)qi);
+                out[t] = val;
             }
+        } else if (threadIdx.x == 0) {
+            flag(a.st, sid, kStatusOverflow);
         }
         __syncthreads();
     }
 }
 
 // ---------------------------------------------------------------------------
-// Grid update: grid_velocity (grid.cpp:195-204) then grid_collisions
-// (grid.cpp:233-243), in FP64, plus the PIC/FLIP blend u that G2P needs:
+// Grid update: each node's mass, momentum and force are the sum, in the fixed
+// order t = 0..7, of the partials of the home blocks B - t that reach it;
+// then grid_velocity (grid.cpp:195-204) and grid_collisions
+// (grid.cpp:233-243) in FP64, plus the PIC/FLIP blend u that G2P needs:
 // u = (1-alpha) v + alpha [m>0] (v - p/m)   (grid.cpp:312-324).
-// One wave per block; consumes and clears the fixed-point accumulators.
+// One wave per block.  With mass_only (setup) only the mass is produced, in
+// the first velocity channel.
 __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lx = (int)(lane & 3), ly = (int)((lane >> 2) & 3), lz = (int)(lane >> 4);
     const uint32_t na = a.cnt->n_all;
     for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
         const uint32_t slot = a.g.all_list[b];
@@ -524,41 +525,45 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
         const SceneDev& sc = a.scenes[sid];
-        unsigned long long* acc = a.g.acc + (size_t)slot * kGridCh * 64;
-        long long q[kGridCh];
+        float q[kGridCh];
 #pragma unroll
-        for (int c = 0; c < kGridCh; ++c) {
-            q[c] = (long long)acc[c * 64 + lane];
-            acc[c * 64 + lane] = 0ull;
+        for (int c = 0; c < kGridCh; ++c) q[c] = 0.0f;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int X = 4 * (t & 1) + lx, Y = 4 * ((t >> 1) & 1) + ly, Z = 4 * (t >> 2) + lz;
+            const uint32_t hb = a.g.rev[(size_t)slot * 8 + t];
+            if (hb != kInvalid && X <= 6 && Y <= 6 && Z <= 6) {
+                const float* src = a.g.part + (size_t)hb * kGridCh * kRegion + (Z * 7 + Y) * 7 + X;
+#pragma unroll
+                for (int c = 0; c < kGridCh; ++c) q[c] += src[c * kRegion];
+            }
         }
+        float* out = a.g.vu + (size_t)slot * kVelCh * 64;
+        if (a.mass_only) {
+            out[lane] = q[0];
+            continue;
+        }
+        if (!isfinite(q[0] + q[1] + q[2] + q[3] + q[4] + q[5] + q[6])) flag(a.st, sid, kStatusNaN);
         double v[3] = {0.0, 0.0, 0.0}, dv[3] = {0.0, 0.0, 0.0};
-        if (a.mass_only == 0 && q[0] > 0) {
-            const double m = (double)q[0] * sc.inv_scale_m;
-            const double im = 1.0 / m;
+        if (q[0] > 0.0f) {
+            const double im = 1.0 / (double)q[0];
             const double dt = sc.dt;
 #pragma unroll
             for (int r = 0; r < 3; ++r) {
-                const double p = (double)q[1 + r] * sc.inv_scale_p;
-                const double f = (double)q[4 + r] * sc.inv_scale_f;
-                const double vold = p * im;
-                v[r] = vold + dt * (f * im + (double)sc.gravity[r]);
+                const double vold = (double)q[1 + r] * im;
+                v[r] = vold + dt * ((double)q[4 + r] * im + (double)sc.gravity[r]);
                 dv[r] = -vold;
             }
         }
-        if (a.mass_only == 0) {
-            const double dx = sc.dx;
-            const double xn0 = dx * (double)(4 * bx + (int)(lane & 3));
-            const double xn1 = dx * (double)(4 * by + (int)((lane >> 2) & 3));
-            const double xn2 = dx * (double)(4 * bz + (int)(lane >> 4));
-            collide<double>(sc, xn0, xn1, xn2, v[0], v[1], v[2]);
-            const double al = sc.alpha;
-            float* out = a.g.vu + (size_t)slot * kVelCh * 64;
+        const double dx = sc.dx;
+        const double xn0 = dx * (double)(4 * bx + lx), xn1 = dx * (double)(4 * by + ly), xn2 = dx * (double)(4 * bz + lz);
+        collide<double>(sc, xn0, xn1, xn2, v[0], v[1], v[2]);
+        const double al = sc.alpha;
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const double flip = (q[0] > 0) ? (v[r] + dv[r]) : 0.0;
-                out[r * 64 + lane] = (float)v[r];
-                out[(3 + r) * 64 + lane] = (float)((1.0 - al) * v[r] + al * flip);
-            }
+        for (int r = 0; r < 3; ++r) {
+            const double flip = (q[0] > 0.0f) ? (v[r] + dv[r]) : 0.0;
+            out[r * 64 + lane] = (float)v[r];
+            out[(3 + r) * 64 + lane] = (float)((1.0 - al) * v[r] + al * flip);
         }
     }
 }
@@ -1035,8 +1040,13 @@ __global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
 // Clear the hash slots used this pass; publish the new particle count.
 __global__ __launch_bounds__(256) void k_end(StepArgs a, int publish) {
     const uint32_t na = a.cnt->n_all;
-    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < na; b += gridDim.x * blockDim.x)
-        a.g.keys[a.g.all_list[b]] = kEmptyKey;
+    for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < na; b += gridDim.x * blockDim.x) {
+        const uint32_t slot = a.g.all_list[b];
+        a.g.keys[slot] = kEmptyKey;
+        uint4* r = reinterpret_cast<uint4*>(a.g.rev + (size_t)slot * 8);
+        r[0] = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+        r[1] = make_uint4(kInvalid, kInvalid, kInvalid, kInvalid);
+    }
     if (publish && blockIdx.x == 0 && threadIdx.x == 0) a.cnt->n_particles = a.cnt->cursor;
 }
 
@@ -1058,8 +1068,7 @@ __global__ __launch_bounds__(256) void k_setup_gather(StepArgs a) {
             const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
             const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
                       Z = (tgt >> 2) * 4 + (nn >> 4);
-            const long long q = (long long)a.g.acc[((size_t)ts * kGridCh + 0) * 64 + nn];
-            ms[Z * 64 + Y * 8 + X] = (float)((double)q * sc.inv_scale_m);
+            ms[Z * 64 + Y * 8 + X] = a.g.vu[((size_t)ts * kVelCh + 0) * 64 + nn];
         }
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
@@ -1106,17 +1115,6 @@ __global__ __launch_bounds__(256) void k_setup_gather(StepArgs a) {
             for (int e = 0; e < 6; ++e) a.cur.S[e][i] = po.S[e];
         }
         __syncthreads();
-    }
-}
-
-// Clears the accumulators of every block used by a mass-only pass.
-__global__ __launch_bounds__(256) void k_clear_acc(StepArgs a) {
-    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const uint32_t na = a.cnt->n_all;
-    for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
-        const uint32_t slot = a.g.all_list[b];
-#pragma unroll
-        for (int c = 0; c < kGridCh; ++c) a.g.acc[((size_t)slot * kGridCh + c) * 64 + lane] = 0ull;
     }
 }
 
@@ -1190,47 +1188,6 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
         for (int e = 0; e < 7; ++e) out7[blockIdx.x * 7 + e] = red[e][0];
 }
 
-// Per-scene maximum of a positive quantity (float bits compare as uints),
-// reduced per thread (while the scene id stays the same) and per wave before
-// the atomics.
-__global__ __launch_bounds__(256) void k_scene_max(StepArgs a, int what) {
-    const uint32_t n = a.cnt->n_particles;
-    const uint32_t lane = threadIdx.x & 63;
-    uint32_t cur_sid = kInvalid;
-    float best = 0.f;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
-        const uint32_t i = wbase + lane;
-        uint32_t sid = kInvalid;
-        float v = 0.f;
-        if (i < n) {
-            sid = a.cur.sid[i];
-            const SceneDev& sc = a.scenes[sid];
-            if (what == 0) {
-                float mu0, la0;
-                lame(sc, a.cur, i, mu0, la0);
-                v = a.cur.V0[i] * (mu0 + la0);
-            }
-            if (!(v > 0.f) || !isfinite(v)) v = 0.f;
-        }
-        if (sid != cur_sid) {
-            if (cur_sid != kInvalid && best > 0.f) atomicMax((unsigned int*)&a.st.vmax[cur_sid], __float_as_uint(best));
-            cur_sid = sid;
-            best = 0.f;
-        }
-        best = fmaxf(best, v);
-    }
-    // wave reduction when the whole wave holds one scene
-    const uint32_t s0 = __shfl(cur_sid, 0);
-    if (__all(cur_sid == s0)) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) best = fmaxf(best, __shfl_xor(best, o));
-        if (lane == 0 && s0 != kInvalid && best > 0.f) atomicMax((unsigned int*)&a.st.vmax[s0], __float_as_uint(best));
-    } else if (cur_sid != kInvalid && best > 0.f) {
-        atomicMax((unsigned int*)&a.st.vmax[cur_sid], __float_as_uint(best));
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Host launch wrappers.
 
@@ -1267,8 +1224,10 @@ void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     k_p2g<<<c.grid_w, 256, 0, s>>>(a);
+    StepArgs am = a;
+    am.mass_only = 1;
+    k_grid<<<c.grid_b, 256, 0, s>>>(am);
     k_setup_gather<<<c.grid_w, 256, 0, s>>>(a);
-    k_clear_acc<<<c.grid_b, 256, 0, s>>>(a);
     k_end<<<c.grid_b, 256, 0, s>>>(a, 0);
 }
 
@@ -1289,8 +1248,5 @@ void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks
     k_scene_sums<<<nblocks, 256, 0, s>>>(a, sid, out);
 }
 
-void launch_scene_max(const StepArgs& a, const LaunchCfg& c, int what, hipStream_t s) {
-    k_scene_max<<<c.grid_p, 256, 0, s>>>(a, what);
-}
 
 }  // namespace snowmpm

@@ -41,6 +41,5 @@ void launch_set_count(Counters* c, uint32_t n, hipStream_t s);
 void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, float* out, uint32_t nscene, int track,
                          hipStream_t s);
 void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s);
-void launch_scene_max(const StepArgs& a, const LaunchCfg& c, int what, hipStream_t s);
 
 }  // namespace snowmpm
