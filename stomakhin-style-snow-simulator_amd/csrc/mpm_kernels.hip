@@ -366,6 +366,18 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // retire in order, so the float sums are deterministic.  The four wave
 // regions are then summed in fixed order, converted to 64-bit fixed point
 // and added to the owning blocks with integer atomics (order independent).
+// A home block H's 7x7x7 node region splits into the pieces that fall into
+// the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
+// bit 0, 3 with bit 1.  part stores piece t at node offset piece_base(t)
+// (times 7 channels), channel-major, nodes x-fastest.
+__host__ __device__ constexpr int piece_n(int t, int axis) { return ((t >> axis) & 1) ? 3 : 4; }
+__host__ __device__ constexpr int piece_base(int t) {
+    int b = 0;
+    for (int u = 0; u < t; ++u) b += piece_n(u, 0) * piece_n(u, 1) * piece_n(u, 2);
+    return b;
+}
+static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
+
 constexpr int kP2GNS = 12;                 // floats per node in a deposit region (7 used; 12 spreads banks)
 constexpr int kP2GW = 4 * 49 * kP2GNS;     // one wave region: 4 z-planes x 7 x 7 nodes
 constexpr int kP2GChunk = 512;             // particles staged per pass (>= a typical block)
@@ -485,20 +497,27 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
             }
         __syncthreads();
 
-        // Partial sums of the 7x7x7 node region: the four wave regions added in
-        // fixed order, stored with plain coalesced writes.
+        // Partial sums of the 7x7x7 node region, the four wave regions added
+        // in fixed order, stored as the eight pieces that fall into the
+        // blocks H + t (layout at piece_n) with coalesced writes.
         if (blk < a.g.max_home) {
             float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
-            for (int t = threadIdx.x; t < kGridCh * kRegion; t += 256) {
-                const int c = t / kRegion, node = t - c * kRegion;
-                const int Z = node / 49, Y = (node / 7) % 7, X = node % 7;
-                float val = 0.0f;
 #pragma unroll
-                for (int ww = 0; ww < 4; ++ww) {
-                    const int zl = Z - ww;
-                    if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
+            for (int t = 0; t < 8; ++t) {
+                const int nx = piece_n(t, 0), ny = piece_n(t, 1), sz = nx * ny * piece_n(t, 2);
+                float* po = out + kGridCh * piece_base(t);
+                for (int o = threadIdx.x; o < kGridCh * sz; o += 256) {
+                    const int c = o / sz, idx = o - c * sz;
+                    const int X = 4 * (t & 1) + idx % nx, Y = 4 * ((t >> 1) & 1) + (idx / nx) % ny,
+                              Z = 4 * (t >> 2) + idx / (nx * ny);
+                    float val = 0.0f;
+#pragma unroll
+                    for (int ww = 0; ww < 4; ++ww) {
+                        const int zl = Z - ww;
+                        if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
+                    }
+                    po[o] = val;
                 }
-                out[t] = val;
             }
         } else if (threadIdx.x == 0) {
             flag(a.st, sid, kStatusOverflow);
@@ -530,12 +549,13 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
         for (int c = 0; c < kGridCh; ++c) q[c] = 0.0f;
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int X = 4 * (t & 1) + lx, Y = 4 * ((t >> 1) & 1) + ly, Z = 4 * (t >> 2) + lz;
+            const int nx = piece_n(t, 0), ny = piece_n(t, 1), nz = piece_n(t, 2), sz = nx * ny * nz;
             const uint32_t hb = a.g.rev[(size_t)slot * 8 + t];
-            if (hb != kInvalid && X <= 6 && Y <= 6 && Z <= 6) {
-                const float* src = a.g.part + (size_t)hb * kGridCh * kRegion + (Z * 7 + Y) * 7 + X;
+            if (hb != kInvalid && lx < nx && ly < ny && lz < nz) {
+                const float* src =
+                    a.g.part + (size_t)hb * kGridCh * kRegion + kGridCh * piece_base(t) + (lz * ny + ly) * nx + lx;
 #pragma unroll
-                for (int c = 0; c < kGridCh; ++c) q[c] += src[c * kRegion];
+                for (int c = 0; c < kGridCh; ++c) q[c] += src[c * sz];
             }
         }
         float* out = a.g.vu + (size_t)slot * kVelCh * 64;
