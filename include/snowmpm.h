@@ -37,7 +37,7 @@ extern "C" {
 
 /* device status bits, per scene */
 #define MPM_STATUS_NAN 0x1u       /* non-finite particle state */
-#define MPM_STATUS_OVERFLOW 0x2u  /* fixed-point grid accumulator overflow */
+#define MPM_STATUS_OVERFLOW 0x2u  /* more occupied blocks than the P2G partial-sum buffer holds */
 #define MPM_STATUS_HASH_FULL 0x4u /* block hash table full */
 #define MPM_STATUS_RANGE 0x8u     /* particle outside the block-coordinate range */
 
