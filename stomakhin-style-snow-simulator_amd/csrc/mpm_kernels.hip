@@ -83,7 +83,19 @@ __global__ void k_begin(StepArgs a) {
         a.cnt->n_all = 0;
         a.cnt->cursor = 0;
         a.cnt->n_fallback = 0;
+        a.cnt->next_p2g = 0;
+        a.cnt->next_g2p = 0;
     }
+}
+
+// Workgroups of P2G and G2P take home blocks from a shared counter, so the
+// launch balances itself whatever the grid size and block sizes (each block's
+// result is independent of which workgroup computes it).  The caller's loop
+// body must end with a barrier before the next fetch.
+__device__ __forceinline__ uint32_t fetch_block(uint32_t* next, uint32_t* s_blk) {
+    if (threadIdx.x == 0) *s_blk = atomicAdd(next, 1u);
+    __syncthreads();
+    return *s_blk;
 }
 
 __global__ void k_set_count(Counters* c, uint32_t n) {
@@ -378,7 +390,7 @@ __host__ __device__ constexpr int piece_base(int t) {
 }
 static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
 
-constexpr int kP2GNS = 12;                 // floats per node in a deposit region (7 used; 12 spreads banks)
+constexpr int kP2GNS = 8;                  // floats per node in a deposit region (7 used)
 constexpr int kP2GW = 4 * 49 * kP2GNS;     // one wave region: 4 z-planes x 7 x 7 nodes
 constexpr int kP2GChunk = 512;             // particles staged per pass (>= a typical block)
 constexpr int kP2GStride = kP2GChunk + kP2GChunk / 32;  // per-field stride after the k + k/32 swizzle
@@ -391,9 +403,13 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
     __shared__ float pst[kP2GFields * kP2GStride];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
+    __shared__ uint32_t s_blk;
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
-        for (int t = threadIdx.x; t < 4 * kP2GW; t += 256) lds[t] = 0.0f;
+    for (;;) {
+        const uint32_t blk = fetch_block(&a.cnt->next_p2g, &s_blk);
+        if (blk >= nh) break;
+        for (int t = threadIdx.x; t < kP2GW; t += 256)
+            reinterpret_cast<float4*>(lds)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         const uint32_t slot = a.g.home_list[blk];
         uint32_t sid;
         int bx, by, bz;
@@ -856,8 +872,11 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 template <bool TRACK>
 __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
     __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
+    __shared__ uint32_t s_blk;
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
+    for (;;) {
+        const uint32_t blk = fetch_block(&a.cnt->next_g2p, &s_blk);
+        if (blk >= nh) break;
         const uint32_t slot = a.g.home_list[blk];
         uint32_t sid;
         int bx, by, bz;
