@@ -47,7 +47,7 @@ def cpu_baseline(seed):
     cs = mpmref.orc_colliders(sc["colliders"])
     op = mpmref.orc_params(prm)
     st.run(cs, op, 1, nthreads=threads)  # warm-up step
-    steps = 10
+    steps = 15
     t0 = time.perf_counter()
     st.run(cs, op, steps, nthreads=threads)
     dt = time.perf_counter() - t0
@@ -173,7 +173,9 @@ def main():
     sc = snowmpm.generate_scene(2, seed=args.seed, scale=1.0)
     n = sc["x"].shape[0]
     eng = snowmpm.Engine(capacity=n + 1024)
+    t_up = time.perf_counter()
     sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"], E=sc["E"])
+    t_up = time.perf_counter() - t_up
     eng.setup()
     print(f"# slab: {n} particles resident, setup {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     if args.warmup:
@@ -184,6 +186,9 @@ def main():
     wall = time.perf_counter() - t0
     kt = eng.kernel_times()
     st, fb = eng.status(sid)
+    t_down = time.perf_counter()
+    eng.download(sid)
+    t_down = time.perf_counter() - t_down
     eng.close()
 
     ms_step = wall * 1e3 / args.steps
@@ -218,6 +223,12 @@ def main():
         "kernel_ms": {k: round(v, 4) for k, v in kt.items() if k != "steps"},
         "status": st,
         "fp64_fallbacks": fb,
+        # Host buffers crossing PCIe (not part of value): scene upload from double
+        # host arrays, state download to double host arrays, and the rate when
+        # every 80-step frame is read back, as the reference driver does.
+        "host_io": {"upload_s": t_up, "download_s": t_down,
+                    "particle_steps_per_s_with_frame_download":
+                        n * 80 / (80 * ms_step * 1e-3 + t_down)},
     }
     if not args.no_serving:
         out["batched"] = batched_and_served(args.seed)
