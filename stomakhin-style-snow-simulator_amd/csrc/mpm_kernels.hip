@@ -865,6 +865,27 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 }
 
 // ---------------------------------------------------------------------------
+// G2P grid staging: neighbour block t of a home block (t = 0..7) is staged by
+// the 32 threads 32 t .. 32 t + 31, each holding 12 of its 6 x 64 floats, so
+// one slot lookup per thread suffices and the loads are 128-byte runs.
+constexpr int kG2PStage = kVelCh * 64 / 32;
+static_assert(kG2PStage * 32 == kVelCh * 64, "staging split");
+
+__device__ __forceinline__ void g2p_stage_load(const StepArgs& a, uint32_t ts, int l32, float (&r)[kG2PStage]) {
+    const float* src = a.g.vu + (size_t)ts * kVelCh * 64;
+#pragma unroll
+    for (int i = 0; i < kG2PStage; ++i) r[i] = src[l32 + 32 * i];
+}
+
+__device__ __forceinline__ void g2p_stage_store(float* vu, int tg, int l32, const float (&r)[kG2PStage]) {
+#pragma unroll
+    for (int i = 0; i < kG2PStage; ++i) {
+        const int e = l32 + 32 * i, c = e >> 6, nn = e & 63;
+        const int X = (tg & 1) * 4 + (nn & 3), Y = ((tg >> 1) & 1) * 4 + ((nn >> 2) & 3), Z = (tg >> 2) * 4 + (nn >> 4);
+        vu[(Z * 64 + Y * 8 + X) * 8 + c + (c >= 3 ? 1 : 0)] = r[i];
+    }
+}
+
 // Fused stress_update + G2P + particle collisions + advection.  One workgroup
 // per home block stages the 8x8x8 nodes of its 8 blocks (v and u) in LDS,
 // then each thread updates one particle and writes it to the next buffer at
@@ -872,27 +893,39 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 template <bool TRACK>
 __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
     __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
-    __shared__ uint32_t s_blk;
     const uint32_t nh = a.cnt->n_home;
-    for (;;) {
-        const uint32_t blk = fetch_block(&a.cnt->next_g2p, &s_blk);
-        if (blk >= nh) break;
-        const uint32_t slot = a.g.home_list[blk];
+    // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
+    // while block blk is computed, the grid data of blk + G is in flight into
+    // registers and the neighbour slots of blk + 2G are being looked up, so the
+    // dependent loads home_list -> nb -> vu never stall the block start.
+    // Threads 32 tg .. 32 tg + 31 stage neighbour block tg (12 floats each).
+    const uint32_t G = gridDim.x;
+    const int tg = (int)(threadIdx.x >> 5), l32 = (int)(threadIdx.x & 31);
+    uint32_t blk = blockIdx.x;
+    float stg[kG2PStage];
+    uint32_t slot = 0, slot1 = 0, ts1 = 0;
+    if (blk < nh) {
+        slot = a.g.home_list[blk];
+        g2p_stage_load(a, a.g.nb[(size_t)slot * 8 + tg], l32, stg);
+    }
+    if (blk + G < nh) {
+        slot1 = a.g.home_list[blk + G];
+        ts1 = a.g.nb[(size_t)slot1 * 8 + tg];
+    }
+    for (; blk < nh; blk += G) {
+        g2p_stage_store(vu, tg, l32, stg);
         uint32_t sid;
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
         const SceneDev& sc = a.scenes[sid];
-        for (int t = threadIdx.x; t < 8 * kVelCh * 64; t += 256) {
-            const int tgt = t / (kVelCh * 64);
-            const int rem = t - tgt * (kVelCh * 64);
-            const int c = rem >> 6, nn = rem & 63;
-            const uint32_t ts = a.g.nb[(size_t)slot * 8 + tgt];
-            const int X = (tgt & 1) * 4 + (nn & 3), Y = ((tgt >> 1) & 1) * 4 + ((nn >> 2) & 3),
-                      Z = (tgt >> 2) * 4 + (nn >> 4);
-            vu[(Z * 64 + Y * 8 + X) * 8 + c + (c >= 3 ? 1 : 0)] = a.g.vu[((size_t)ts * kVelCh + c) * 64 + nn];
-        }
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
+        uint32_t slot2 = 0, ts2 = 0;
+        if (blk + G < nh) g2p_stage_load(a, ts1, l32, stg);
+        if (blk + 2 * G < nh) {
+            slot2 = a.g.home_list[blk + 2 * G];
+            ts2 = a.g.nb[(size_t)slot2 * 8 + tg];
+        }
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
         const float inv_dx = sc.inv_dx, dt = sc.dt, alpha = sc.alpha;
         for (uint32_t j = off + threadIdx.x; j < off + cnt; j += 256) {
@@ -1040,6 +1073,9 @@ __global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
             a.nxt.sid[j] = sid;
         }
         __syncthreads();
+        slot = slot1;
+        slot1 = slot2;
+        ts1 = ts2;
     }
 }
 
