@@ -200,3 +200,26 @@ def test_prestressed_elastic_matches_oracle():
     assert dv < 1e-3
     assert np.abs(got["Fe"] - ref["Fe"]).max() < 1e-5
     assert np.abs(got["x"] - ref["x"]).max() < 1e-6
+
+
+@pytest.mark.parametrize("track_fp", [False, True])
+def test_golden_trajectory(track_fp):
+    """The committed oracle trajectory (tests/golden/oracle_trajectory.npz):
+    1,000 particles in a ground/tilted-plane wedge, setup + 10 steps."""
+    import os
+    ref = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_trajectory.npz"))
+    s = 2 ** -0.5
+    prm = snowmpm.params(1e-4, 0.02, *snowmpm.lame(1.4e5, 0.2), 10.0, 0.025, 0.0075, 0.95)
+    sc = dict(x=ref["x0"], v=ref["v0"], m=ref["m"], params=prm,
+              colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2),
+                         snowmpm.make_plane((0.09, 0, 0), (-s, s, 0), 0.5)])
+    got = mpmref.run_gpu(sc, 10, track_fp=track_fp)
+    ex, ev = mpmref.rel_errors(got, dict(x=ref["x_10"], v=ref["v_10"]), sc)
+    print(f"golden trajectory (track_fp={track_fp}): rel err x {ex:.3e} v {ev:.3e}")
+    assert ex < TOL and ev < TOL
+    assert np.abs(got["V0"] - ref["V0"]).max() / ref["V0"].max() < 1e-5
+    assert np.abs(got["Fe"] - ref["Fe_10"].reshape(-1, 3, 3)).max() < 1e-4
+    assert np.abs(got["Jp"] - np.linalg.det(ref["Fp_10"].reshape(-1, 3, 3))).max() < 1e-4
+    if track_fp:
+        assert np.abs(got["Fp"] - ref["Fp_10"].reshape(-1, 3, 3)).max() < 1e-4
+    assert got["status"][0] == 0
