@@ -890,8 +890,10 @@ __device__ __forceinline__ void g2p_stage_store(float* vu, int tg, int l32, cons
 // per home block stages the 8x8x8 nodes of its 8 blocks (v and u) in LDS,
 // then each thread updates one particle and writes it to the next buffer at
 // its sorted position.
+// Four waves per SIMD: the register budget (<= 128 VGPRs) is met without
+// spills and the extra resident waves hide the particle and grid loads.
 template <bool TRACK>
-__global__ __launch_bounds__(256) void k_g2p(StepArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
     __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
     const uint32_t nh = a.cnt->n_home;
     // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
