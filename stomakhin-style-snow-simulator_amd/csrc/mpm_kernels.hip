@@ -376,8 +376,7 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // particle.  Deposits go to the wave's private LDS region; for a fixed loop
 // index every lane of a wave hits a different node, and a wave's LDS ops
 // retire in order, so the float sums are deterministic.  The four wave
-// regions are then summed in fixed order, converted to 64-bit fixed point
-// and added to the owning blocks with integer atomics (order independent).
+// regions are then summed in fixed order into the block's partial sums.
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
 // bit 0, 3 with bit 1.  part stores piece t at node offset piece_base(t)
@@ -482,9 +481,10 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
                         acc[aa][bb][1] += wa * q0;
                         acc[aa][bb][2] += wa * q1;
                         acc[aa][bb][3] += wa * q2;
-                        acc[aa][bb][4] += da * B0 + wa * A0;
-                        acc[aa][bb][5] += da * B1 + wa * A1;
-                        acc[aa][bb][6] += da * B2 + wa * A2;
+                        // two chained FMAs (acc + (da B + wa A) would cost an extra add)
+                        acc[aa][bb][4] = fmaf(wa, A0, fmaf(da, B0, acc[aa][bb][4]));
+                        acc[aa][bb][5] = fmaf(wa, A1, fmaf(da, B1, acc[aa][bb][5]));
+                        acc[aa][bb][6] = fmaf(wa, A2, fmaf(da, B2, acc[aa][bb][6]));
                     }
                 }
             }
