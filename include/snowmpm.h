@@ -91,7 +91,9 @@ typedef struct {
     int hash_log2;            /* block hash slots = 2^hash_log2; 0 = sized from capacity */
     int track_fp;             /* 1: keep the full plastic gradient F_P; 0: keep J_P = det F_P */
     int device;               /* HIP device ordinal */
-    int pad_;
+    int permute_period;       /* steps between full re-sorts of the particle buffers; 0 = 16.
+                                 Results do not depend on it (steps in between update the
+                                 particles at their index); 1 re-sorts every step */
     double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
 } mpm_config;
 

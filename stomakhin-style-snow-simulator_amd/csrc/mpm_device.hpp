@@ -1,9 +1,10 @@
 // mpm_device.hpp -- device-side data layout of the MI355X MPM step engine.
 //
-// Particles live in HBM as FP32 structure-of-arrays, double buffered: the
-// G2P kernel reads buffer `cur` through the block-sorted index and writes
-// buffer `cur^1` in sorted order, so every step leaves particles grouped by
-// their 4^3-node home block.  The grid is block sparse: a device hash maps a
+// Particles live in HBM as FP32 structure-of-arrays, double buffered.  G2P
+// reads the current state through the block-sorted index and writes the
+// fields a step changes to the other buffer at the same index; every few
+// steps it instead writes whole particles in sorted order, so the buffers stay
+// grouped by 4^3-node home block.  The grid is block sparse: a device hash maps a
 // packed (scene, bx, by, bz) key to a slot, and the slot index is the block's
 // storage index in every per-block array (no indirection, no compaction).
 #pragma once
@@ -71,6 +72,18 @@ struct PBuf {
     uint32_t* pid; // particle index within its scene (download order, sort tie-break)
     uint32_t* sid; // scene id
 };
+
+// The fields a step changes (x v F_E J_P F_P S) and the ones it does not
+// (m V0 E pid sid) of a particle view may come from different buffers.
+inline PBuf compose_pbuf(const PBuf& moving, const PBuf& constant) {
+    PBuf p = moving;
+    p.m = constant.m;
+    p.V0 = constant.V0;
+    p.E = constant.E;
+    p.pid = constant.pid;
+    p.sid = constant.sid;
+    return p;
+}
 
 struct Counters {
     uint32_t n_particles;  // particles in the current buffer

@@ -64,6 +64,11 @@ int Engine::init(const mpm_config& cfg) {
         return MPM_ERR_ARG;
     }
     if (cfg_.kappa <= 0) cfg_.kappa = 1.5;
+    if (cfg_.permute_period < 0) {
+        set_error("permute_period < 0");
+        return MPM_ERR_ARG;
+    }
+    if (cfg_.permute_period == 0) cfg_.permute_period = 16;
     HIP_TRY(hipSetDevice(cfg_.device));
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     int cus = 256;
@@ -153,11 +158,11 @@ int Engine::init(const mpm_config& cfg) {
     return MPM_OK;
 }
 
-StepArgs Engine::args(uint32_t pass_flag) const {
+StepArgs Engine::args(uint32_t pass_flag, int permute) const {
     StepArgs a;
     std::memset(&a, 0, sizeof a);
-    a.cur = buf_[cur_];
-    a.nxt = buf_[cur_ ^ 1];
+    a.cur = view(mov_, con_);
+    a.nxt = view(mov_ ^ 1, permute ? con_ ^ 1 : con_);
     a.g = grid_;
     a.bin = bin_;
     a.st = st_;
@@ -168,7 +173,15 @@ StepArgs Engine::args(uint32_t pass_flag) const {
     a.mass_only = 0;
     a.copy_E = any_E_ ? 1 : 0;
     a.kappa = (float)cfg_.kappa;
+    a.permute = permute;
     return a;
+}
+
+bool Engine::take_permute() {
+    const bool p = need_permute_ || (step_count_ % (uint64_t)cfg_.permute_period) == 0;
+    need_permute_ = false;
+    ++step_count_;
+    return p;
 }
 
 int Engine::device_count(uint32_t* n) {
@@ -243,7 +256,7 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
         stage[o + i] = E ? (float)E[i] : 0.0f;
         pid[i] = (uint32_t)i;
     }
-    const PBuf& p = buf_[cur_];
+    const PBuf p = view(mov_, con_);
     {
         size_t o = 0;
         auto put = [&](float* dst) -> hipError_t {
@@ -308,6 +321,7 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     sh.n = n;
     sh.prm = prm;
     *out_sid = sid;
+    need_permute_ = true;  // bring the appended particles into sorted order
     return MPM_OK;
 }
 
@@ -316,13 +330,15 @@ int Engine::scene_remove(int sid) {
         set_error("bad scene id");
         return MPM_ERR_ARG;
     }
-    // Particles of a scene with no flags are dropped by the next binning pass
-    // (G2P only writes binned particles), which compacts the buffer.
+    // Particles of a scene with no flags are not binned; the next step is a
+    // permuting one, whose G2P writes only binned particles and so compacts
+    // the buffer.
     h_scenes_[sid].flags = 0;
     int rc = upload_scene_table(sid);
     if (rc) return rc;
     scenes_[sid].used = false;
     scenes_[sid].pending_drop = true;
+    need_permute_ = true;
     return MPM_OK;
 }
 
@@ -334,6 +350,7 @@ int Engine::clear() {
     }
     scenes_.clear();
     any_E_ = false;
+    need_permute_ = true;
     launch_set_count(cnt_, 0, stream_);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream_));
@@ -391,9 +408,9 @@ int Engine::step(int nsteps, mpm_timings* t) {
         }
     }
     for (int s = 0; s < nsteps; ++s) {
-        StepArgs a = args(kSceneActive);
+        const StepArgs a = args(kSceneActive, take_permute() ? 1 : 0);
         launch_step(a, launch_, stream_, timed ? &events_[(size_t)s * kStepEvents] : nullptr);
-        cur_ ^= 1;
+        advance(a.permute);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(stream_));

@@ -26,7 +26,7 @@ void set_error(const std::string& s);
 
 struct SceneHost {
     bool used = false;
-    bool pending_drop = false;  // removed; particles leave at the next step
+    bool pending_drop = false;  // removed; particles leave at the next (permuting) step
     size_t n = 0;
     mpm_params prm{};
 };
@@ -65,11 +65,17 @@ class Engine {
   private:
     int dalloc(void** p, size_t bytes);
     void release();
-    StepArgs args(uint32_t pass_flag) const;
+    StepArgs args(uint32_t pass_flag, int permute = 1) const;
+    PBuf view(int mov, int con) const { return compose_pbuf(buf_[mov], buf_[con]); }
+    void advance(int permute) {
+        mov_ ^= 1;
+        if (permute) con_ ^= 1;
+    }
     int device_count(uint32_t* n);
     int upload_scene_table(int sid);
     int check_status();
-    int ensure_graph(int parity);
+    int ensure_graph(int permute);
+    bool take_permute();
     void drop_graphs();
     double footprint_blocks(size_t n) const;
 
@@ -78,7 +84,11 @@ class Engine {
     LaunchCfg launch_{};
     uint32_t H_ = 0;
     PBuf buf_[2]{};
-    int cur_ = 0;
+    // Current particle state: moving fields in buf_[mov_], constant fields in
+    // buf_[con_].  Every step writes the moving fields to the other buffer;
+    // a permuting step also moves the constant fields.
+    int mov_ = 0;
+    int con_ = 0;
     GridDev grid_{};
     BinDev bin_{};
     FallbackDev fb_{};
@@ -92,9 +102,15 @@ class Engine {
     std::vector<SceneHost> scenes_;
     std::vector<void*> allocs_;
     std::vector<hipEvent_t> events_;
-    hipGraphExec_t gexec_[2] = {nullptr, nullptr};
-    StepArgs gargs_[2]{};
-    LaunchCfg glaunch_[2]{};
+    // Steps update the particles at their index and re-sort the buffers (a
+    // permuting step: particles move to their sorted positions, dropped scenes
+    // are compacted away) every cfg_.permute_period steps, and on the first
+    // step after scenes were added or removed.
+    bool need_permute_ = true;
+    uint64_t step_count_ = 0;
+    hipGraphExec_t gexec_[8] = {};  // [(mov_ * 2 + con_) * 2 + permute]
+    StepArgs gargs_[8]{};
+    LaunchCfg glaunch_[8]{};
     int cus_ = 256;
     double kernel_ms_[5] = {0, 0, 0, 0, 0};
     int kernel_steps_ = 0;

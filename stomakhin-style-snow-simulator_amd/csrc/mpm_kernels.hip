@@ -892,7 +892,7 @@ __device__ __forceinline__ void g2p_stage_store(float* vu, int tg, int l32, cons
 // its sorted position.
 // Four waves per SIMD: the register budget (<= 128 VGPRs) is met without
 // spills and the extra resident waves hide the particle and grid loads.
-template <bool TRACK>
+template <bool TRACK, bool PERMUTE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
     __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
     const uint32_t nh = a.cnt->n_home;
@@ -1045,6 +1045,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     for (int e = 0; e < 9; ++e) Fp_new[e] = Fp_old[e];
             }
 
+            // Permuting pass: the particle moves to sorted position j of the
+            // next buffers with all its fields.  Otherwise only the fields a
+            // step changes are written, at the particle's own index i, into
+            // the other buffer of the moving fields (nxt never aliases cur).
+            // The deformation state is stored first, which frees its registers
+            // before the velocity update.
+            const PBuf& o = a.nxt;
+            const uint32_t w = PERMUTE ? j : i;
+#pragma unroll
+            for (int e = 0; e < 9; ++e) o.F[e][w] = po.Fe[e];
+            o.Jp[w] = po.Jp;
+            if (track) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) o.Fp[e][w] = Fp_new[e];
+            }
+#pragma unroll
+            for (int e = 0; e < 6; ++e) o.S[e][w] = po.S[e];
+
             // v_p = (1-alpha) v_pic + alpha (v_p + sum w (v_i - p_i/m_i))  (grid.cpp:321-324)
             v0 = alpha * v0 + pv0;
             v1 = alpha * v1 + pv1;
@@ -1053,26 +1071,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             const float nx0 = x0 + dt * v0, nx1 = x1 + dt * v1, nx2 = x2 + dt * v2;  // grid.cpp:335-338
             if (!finite3(nx0, nx1, nx2) || !finite3(v0, v1, v2) || !isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
 
-            a.nxt.x[0][j] = nx0;
-            a.nxt.x[1][j] = nx1;
-            a.nxt.x[2][j] = nx2;
-            a.nxt.v[0][j] = v0;
-            a.nxt.v[1][j] = v1;
-            a.nxt.v[2][j] = v2;
-#pragma unroll
-            for (int e = 0; e < 9; ++e) a.nxt.F[e][j] = po.Fe[e];
-            a.nxt.Jp[j] = po.Jp;
-            if (track) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e) a.nxt.Fp[e][j] = Fp_new[e];
+            o.x[0][w] = nx0;
+            o.x[1][w] = nx1;
+            o.x[2][w] = nx2;
+            o.v[0][w] = v0;
+            o.v[1][w] = v1;
+            o.v[2][w] = v2;
+            if (PERMUTE) {
+                a.nxt.m[j] = a.cur.m[i];
+                a.nxt.V0[j] = V0;
+                if (a.copy_E) a.nxt.E[j] = a.cur.E[i];
+                a.nxt.pid[j] = a.cur.pid[i];
+                a.nxt.sid[j] = sid;
             }
-#pragma unroll
-            for (int e = 0; e < 6; ++e) a.nxt.S[e][j] = po.S[e];
-            a.nxt.m[j] = a.cur.m[i];
-            a.nxt.V0[j] = V0;
-            if (a.copy_E) a.nxt.E[j] = a.cur.E[i];
-            a.nxt.pid[j] = a.cur.pid[i];
-            a.nxt.sid[j] = sid;
         }
         __syncthreads();
         slot = slot1;
@@ -1083,7 +1094,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 
 // FP64 redo of the plasticity update for the particles G2P deferred
 // (inverted, singular or cond(trial F_E) > kappa).  Reads the pre-step
-// state through the sorted index, writes the post-step state in place.
+// state through the sorted index (G2P left the old F_E, J_P and F_P there
+// for these particles) and writes the post-step state where G2P wrote.
 __global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
     const uint32_t nf = a.cnt->n_fallback;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x) {
@@ -1102,19 +1114,22 @@ __global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
         float mu0, la0;
         lame(sc, a.cur, i, mu0, la0);
         PlastOut po;
+        const PBuf& o = a.nxt;
+        const uint32_t w = a.permute ? j : i;
         plastic_fp64(Fe, L, sc.dt, a.cur.Jp[i], track ? Fp_old : nullptr, Fp_new, true, sc.theta_c, sc.theta_s, mu0,
                      la0, sc.hardening, a.cur.V0[i], po);
-        for (int e = 0; e < 9; ++e) a.nxt.F[e][j] = po.Fe[e];
-        a.nxt.Jp[j] = po.Jp;
-        for (int e = 0; e < 6; ++e) a.nxt.S[e][j] = po.S[e];
+        for (int e = 0; e < 9; ++e) o.F[e][w] = po.Fe[e];
+        o.Jp[w] = po.Jp;
+        for (int e = 0; e < 6; ++e) o.S[e][w] = po.S[e];
         if (track)
-            for (int e = 0; e < 9; ++e) a.nxt.Fp[e][j] = Fp_new[e];
+            for (int e = 0; e < 9; ++e) o.Fp[e][w] = Fp_new[e];
         if (!isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
         atomicAdd(&a.st.fp64_count[sid], 1ull);
     }
 }
 
-// Clear the hash slots used this pass; publish the new particle count.
+// Clear the hash slots used this pass; after a permuting pass publish the
+// new (compacted) particle count.
 __global__ __launch_bounds__(256) void k_end(StepArgs a, int publish) {
     const uint32_t na = a.cnt->n_all;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < na; b += gridDim.x * blockDim.x) {
@@ -1282,13 +1297,20 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     if (ev) (void)hipEventRecord(ev[2], s);
     k_grid<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
-    if (a.cur.Fp[0])
-        k_g2p<true><<<c.grid_w, 256, 0, s>>>(a);
-    else
-        k_g2p<false><<<c.grid_w, 256, 0, s>>>(a);
+    if (a.cur.Fp[0]) {
+        if (a.permute)
+            k_g2p<true, true><<<c.grid_w, 256, 0, s>>>(a);
+        else
+            k_g2p<true, false><<<c.grid_w, 256, 0, s>>>(a);
+    } else {
+        if (a.permute)
+            k_g2p<false, true><<<c.grid_w, 256, 0, s>>>(a);
+        else
+            k_g2p<false, false><<<c.grid_w, 256, 0, s>>>(a);
+    }
     k_fp64_fixup<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[4], s);
-    k_end<<<c.grid_b, 256, 0, s>>>(a, 1);
+    k_end<<<c.grid_b, 256, 0, s>>>(a, a.permute);
     if (ev) (void)hipEventRecord(ev[5], s);
 }
 

@@ -9,7 +9,8 @@ namespace snowmpm {
 
 struct StepArgs {
     PBuf cur;             // particles read this pass
-    PBuf nxt;             // particles written by G2P (sorted order)
+    PBuf nxt;             // particles written by G2P: all fields at sorted positions
+                          // (permute), else the moving fields at the same index
     GridDev g;
     BinDev bin;
     StatusDev st;
@@ -20,6 +21,8 @@ struct StepArgs {
     int mass_only;
     int copy_E;           // some scene carries per-particle E
     float kappa;          // FP64 fallback bound on cond(trial F_E)
+    int permute;          // 1: G2P writes every particle to nxt in sorted order (and
+                          //    compacts); 0: only x v F_E J_P [F_P] S, at index i
 };
 
 struct LaunchCfg {

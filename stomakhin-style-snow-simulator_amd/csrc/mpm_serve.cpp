@@ -15,28 +15,27 @@
 
 namespace snowmpm {
 
-// One captured graph per buffer parity: the kernels read every count from
-// device memory, so a graph stays valid while scenes come and go; it is
-// re-captured only when the launch arguments or shapes change.
-int Engine::ensure_graph(int parity) {
-    StepArgs a = args(kSceneActive);
-    a.cur = buf_[parity];
-    a.nxt = buf_[parity ^ 1];
-    if (gexec_[parity] && std::memcmp(&a, &gargs_[parity], sizeof a) == 0 &&
-        std::memcmp(&launch_, &glaunch_[parity], sizeof launch_) == 0)
+// One captured graph per (buffer parities, permuting or not): the kernels read
+// every count from device memory, so a graph stays valid while scenes come and
+// go; it is re-captured only when the launch arguments or shapes change.
+int Engine::ensure_graph(int permute) {
+    const int k = (mov_ * 2 + con_) * 2 + permute;
+    const StepArgs a = args(kSceneActive, permute);
+    if (gexec_[k] && std::memcmp(&a, &gargs_[k], sizeof a) == 0 &&
+        std::memcmp(&launch_, &glaunch_[k], sizeof launch_) == 0)
         return MPM_OK;
-    if (gexec_[parity]) {
-        HIP_TRY(hipGraphExecDestroy(gexec_[parity]));
-        gexec_[parity] = nullptr;
+    if (gexec_[k]) {
+        HIP_TRY(hipGraphExecDestroy(gexec_[k]));
+        gexec_[k] = nullptr;
     }
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     launch_step(a, launch_, stream_, nullptr);
     HIP_TRY(hipStreamEndCapture(stream_, &g));
-    HIP_TRY(hipGraphInstantiate(&gexec_[parity], g, nullptr, nullptr, 0));
+    HIP_TRY(hipGraphInstantiate(&gexec_[k], g, nullptr, nullptr, 0));
     HIP_TRY(hipGraphDestroy(g));
-    gargs_[parity] = a;
-    glaunch_[parity] = launch_;
+    gargs_[k] = a;
+    glaunch_[k] = launch_;
     return MPM_OK;
 }
 
@@ -48,9 +47,10 @@ int Engine::step_graph(int nsteps) {
     int rc = setup();
     if (rc) return rc;
     for (int s = 0; s < nsteps; ++s) {
-        if ((rc = ensure_graph(cur_))) return rc;
-        HIP_TRY(hipGraphLaunch(gexec_[cur_], stream_));
-        cur_ ^= 1;
+        const int permute = take_permute() ? 1 : 0;
+        if ((rc = ensure_graph(permute))) return rc;
+        HIP_TRY(hipGraphLaunch(gexec_[(mov_ * 2 + con_) * 2 + permute], stream_));
+        advance(permute);
     }
     HIP_TRY(hipStreamSynchronize(stream_));
     if (nsteps > 0)
@@ -69,8 +69,9 @@ int Engine::autotune(int spc, int chosen[4]) {
         launch_ = c;
         HIP_TRY(hipEventRecord(e0, stream_));
         for (int s = 0; s < spc; ++s) {
-            launch_step(args(kSceneActive), launch_, stream_, nullptr);
-            cur_ ^= 1;
+            const StepArgs a = args(kSceneActive, take_permute() ? 1 : 0);
+            launch_step(a, launch_, stream_, nullptr);
+            advance(a.permute);
         }
         HIP_TRY(hipEventRecord(e1, stream_));
         HIP_TRY(hipEventSynchronize(e1));

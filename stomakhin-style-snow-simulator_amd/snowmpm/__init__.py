@@ -61,7 +61,7 @@ class StepTimings(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("particle_capacity", C.c_size_t), ("hash_log2", C.c_int), ("track_fp", C.c_int),
-                ("device", C.c_int), ("pad_", C.c_int), ("kappa", C.c_double)]
+                ("device", C.c_int), ("permute_period", C.c_int), ("kappa", C.c_double)]
 
 
 class Request(C.Structure):
@@ -218,9 +218,9 @@ def export_frame(directory, frame_no, x):
 class Engine:
     """Device-resident MPM engine holding any number of scenes."""
 
-    def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0):
+    def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0, permute_period=0):
         cfg = Config(particle_capacity=int(capacity), hash_log2=int(hash_log2), track_fp=int(bool(track_fp)),
-                     device=int(device), pad_=0, kappa=float(kappa))
+                     device=int(device), permute_period=int(permute_period), kappa=float(kappa))
         h = C.c_void_p()
         _check(lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
