@@ -93,7 +93,7 @@ struct Counters {
     uint32_t n_fallback;   // particles deferred to the FP64 fix-up this pass
     uint32_t global_status;
     uint32_t next_p2g;     // next home-list index a P2G workgroup takes
-    uint32_t next_g2p;     // next home-list index a G2P workgroup takes
+    uint32_t pad_;
 };
 
 struct GridDev {

@@ -84,14 +84,13 @@ __global__ void k_begin(StepArgs a) {
         a.cnt->cursor = 0;
         a.cnt->n_fallback = 0;
         a.cnt->next_p2g = 0;
-        a.cnt->next_g2p = 0;
     }
 }
 
-// Workgroups of P2G and G2P take home blocks from a shared counter, so the
-// launch balances itself whatever the grid size and block sizes (each block's
-// result is independent of which workgroup computes it).  The caller's loop
-// body must end with a barrier before the next fetch.
+// P2G workgroups take home blocks from a shared counter, so the launch
+// balances itself whatever the grid size and block sizes (each block's result
+// is independent of which workgroup computes it).  The caller's loop body
+// must end with a barrier before the next fetch.
 __device__ __forceinline__ uint32_t fetch_block(uint32_t* next, uint32_t* s_blk) {
     if (threadIdx.x == 0) *s_blk = atomicAdd(next, 1u);
     __syncthreads();
@@ -397,11 +396,58 @@ constexpr int kP2GFields = 13;             // fx fy fz m mv0 mv1 mv2 s00 s11 s22
 
 __device__ __forceinline__ int p2g_swz(int k) { return k + (k >> 5); }
 
+// The z weight of stencil plane d (0..3) as a function of the cell offset f:
+// w = x^2 (c3 x + c2) + c0 and dw/df = x (e2 x + e1), with x = 1 - f for the
+// planes d = 0, 2 and x = f for d = 1, 3 -- the pieces of the cubic B-spline
+// (reference grid.cpp:24-58) that bspline() evaluates for all four planes.
+struct ZWeight {
+    bool use_t;
+    float c3, c2, c0, e2, e1;
+};
+__device__ __forceinline__ ZWeight zweight_for(int d) {
+    ZWeight z;
+    z.use_t = (d == 0 || d == 2);
+    const bool outer = (d == 0 || d == 3);
+    z.c3 = outer ? (1.0f / 6.0f) : 0.5f;
+    z.c2 = outer ? 0.0f : -1.0f;
+    z.c0 = outer ? 0.0f : (2.0f / 3.0f);
+    const float sgn = z.use_t ? -1.0f : 1.0f;
+    z.e2 = sgn * (outer ? 0.5f : 1.5f);
+    z.e1 = sgn * (outer ? 0.0f : -2.0f);
+    return z;
+}
+
+// Flush table: output o of a home block's partial sums (piece t, channel c,
+// node (X, Y, Z) of the 7^3 region) reads wave regions ww = lo..hi (those
+// whose planes w..w+3 contain Z) at LDS index base + ww * kFlushRegionStep.
+constexpr int kFlushRegionStep = kP2GW - 49 * kP2GNS;
+struct FlushTable {
+    uint16_t e[kGridCh * kRegion];
+};
+constexpr FlushTable make_flush_table() {
+    FlushTable t{};
+    for (int pt = 0; pt < 8; ++pt) {
+        const int nx = piece_n(pt, 0), ny = piece_n(pt, 1), nz = piece_n(pt, 2), sz = nx * ny * nz;
+        for (int c = 0; c < kGridCh; ++c)
+            for (int idx = 0; idx < sz; ++idx) {
+                const int X = 4 * (pt & 1) + idx % nx, Y = 4 * ((pt >> 1) & 1) + (idx / nx) % ny,
+                          Z = 4 * (pt >> 2) + idx / (nx * ny);
+                const int base = ((Z * 7 + Y) * 7 + X) * kP2GNS + c;
+                const int lo = Z > 3 ? Z - 3 : 0, hi = Z < 3 ? Z : 3;
+                t.e[kGridCh * piece_base(pt) + c * sz + idx] = (uint16_t)(base | (lo << 12) | (hi << 14));
+            }
+    }
+    return t;
+}
+static_assert(((6 * 7 + 6) * 7 + 6) * kP2GNS + kGridCh < 4096, "flush table packs the LDS index in 12 bits");
+__constant__ FlushTable kFlush = make_flush_table();
+
 __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
     __shared__ __align__(16) float lds[4 * kP2GW];
     __shared__ float pst[kP2GFields * kP2GStride];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
+    const ZWeight zw = zweight_for(d);
     __shared__ uint32_t s_blk;
     const uint32_t nh = a.cnt->n_home;
     for (;;) {
@@ -457,14 +503,13 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
                 const float s00 = pst[7 * kP2GStride + q], s11 = pst[8 * kP2GStride + q], s22 = pst[9 * kP2GStride + q];
                 const float s01 = pst[10 * kP2GStride + q], s02 = pst[11 * kP2GStride + q],
                             s12 = pst[12 * kP2GStride + q];
-                float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+                float wx[4], dwx[4], wy[4], dwy[4];
                 bspline(fx, wx, dwx);
                 bspline(fy, wy, dwy);
-                bspline(fz, wz, dwz);
-                float wzd = wz[0], dwzd = dwz[0];
-                if (d == 1) { wzd = wz[1]; dwzd = dwz[1]; }
-                if (d == 2) { wzd = wz[2]; dwzd = dwz[2]; }
-                if (d == 3) { wzd = wz[3]; dwzd = dwz[3]; }
+                const float xz = zw.use_t ? 1.0f - fz : fz;   // this lane's z plane only
+                const float x2 = xz * xz;
+                const float wzd = x2 * (zw.c3 * xz + zw.c2) + zw.c0;
+                const float dwzd = xz * (zw.e2 * xz + zw.e1);
 #pragma unroll
                 for (int bb = 0; bb < 4; ++bb) {
                     const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
@@ -513,27 +558,20 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
             }
         __syncthreads();
 
-        // Partial sums of the 7x7x7 node region, the four wave regions added
-        // in fixed order, stored as the eight pieces that fall into the
-        // blocks H + t (layout at piece_n) with coalesced writes.
+        // Partial sums of the 7x7x7 node region, stored as the eight pieces
+        // that fall into the blocks H + t (layout at piece_n) with coalesced
+        // writes; each output adds the wave regions holding its node in fixed
+        // order (table kFlush: LDS address and first/last region).
         if (blk < a.g.max_home) {
             float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
+            for (int o = threadIdx.x; o < kGridCh * kRegion; o += 256) {
+                const uint32_t e = kFlush.e[o];
+                const int base = (int)(e & 0xFFFu), lo = (int)((e >> 12) & 3u), hi = (int)(e >> 14);
+                float val = 0.0f;
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int nx = piece_n(t, 0), ny = piece_n(t, 1), sz = nx * ny * piece_n(t, 2);
-                float* po = out + kGridCh * piece_base(t);
-                for (int o = threadIdx.x; o < kGridCh * sz; o += 256) {
-                    const int c = o / sz, idx = o - c * sz;
-                    const int X = 4 * (t & 1) + idx % nx, Y = 4 * ((t >> 1) & 1) + (idx / nx) % ny,
-                              Z = 4 * (t >> 2) + idx / (nx * ny);
-                    float val = 0.0f;
-#pragma unroll
-                    for (int ww = 0; ww < 4; ++ww) {
-                        const int zl = Z - ww;
-                        if (zl >= 0 && zl < 4) val += lds[ww * kP2GW + ((zl * 7 + Y) * 7 + X) * kP2GNS + c];
-                    }
-                    po[o] = val;
-                }
+                for (int ww = 0; ww < 4; ++ww)
+                    if (ww >= lo && ww <= hi) val += lds[base + ww * kFlushRegionStep];
+                out[o] = val;
             }
         } else if (threadIdx.x == 0) {
             flag(a.st, sid, kStatusOverflow);
