@@ -92,8 +92,7 @@ struct Counters {
     uint32_t cursor;       // particles binned this pass
     uint32_t n_fallback;   // particles deferred to the FP64 fix-up this pass
     uint32_t global_status;
-    uint32_t next_p2g;     // next home-list index a P2G workgroup takes
-    uint32_t pad_;
+    uint32_t pad_[2];
 };
 
 struct GridDev {

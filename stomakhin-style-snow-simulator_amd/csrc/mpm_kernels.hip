@@ -83,19 +83,10 @@ __global__ void k_begin(StepArgs a) {
         a.cnt->n_all = 0;
         a.cnt->cursor = 0;
         a.cnt->n_fallback = 0;
-        a.cnt->next_p2g = 0;
     }
 }
 
-// P2G workgroups take home blocks from a shared counter, so the launch
-// balances itself whatever the grid size and block sizes (each block's result
-// is independent of which workgroup computes it).  The caller's loop body
-// must end with a barrier before the next fetch.
-__device__ __forceinline__ uint32_t fetch_block(uint32_t* next, uint32_t* s_blk) {
-    if (threadIdx.x == 0) *s_blk = atomicAdd(next, 1u);
-    __syncthreads();
-    return *s_blk;
-}
+
 
 __global__ void k_set_count(Counters* c, uint32_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->n_particles = n;
@@ -442,30 +433,62 @@ constexpr FlushTable make_flush_table() {
 static_assert(((6 * 7 + 6) * 7 + 6) * kP2GNS + kGridCh < 4096, "flush table packs the LDS index in 12 bits");
 __constant__ FlushTable kFlush = make_flush_table();
 
+// A home block's records as one P2G lane needs them.
+struct P2GHeader {
+    uint64_t key;
+    uint32_t off;   // first sorted position
+    int nb;         // particles in the block
+    int cbeg, cend; // the lane's cell range within the block
+};
+__device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int l, P2GHeader& h) {
+    h.key = a.g.keys[slot];
+    h.off = a.g.blkoff[slot];
+    h.nb = (int)a.g.blkcnt[slot];
+    h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + l];
+    h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + l];
+}
+// Particle indices of the first staging chunk (sorted positions t and t + 256).
+__device__ __forceinline__ void p2g_first_indices(const StepArgs& a, const P2GHeader& h, uint32_t& i0, uint32_t& i1) {
+    const int t = (int)threadIdx.x, cn = min(kP2GChunk, h.nb);
+    i0 = t < cn ? a.bin.sidx[h.off + t] : 0u;
+    i1 = t + 256 < cn ? a.bin.sidx[h.off + t + 256] : 0u;
+}
+
 __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
     __shared__ __align__(16) float lds[4 * kP2GW];
     __shared__ float pst[kP2GFields * kP2GStride];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
     const ZWeight zw = zweight_for(d);
-    __shared__ uint32_t s_blk;
+    const int l = cx + 4 * cy + 16 * w;
     const uint32_t nh = a.cnt->n_home;
-    for (;;) {
-        const uint32_t blk = fetch_block(&a.cnt->next_p2g, &s_blk);
-        if (blk >= nh) break;
+    // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
+    // the next block's header is loaded while this block's particles are
+    // accumulated, and its first sorted indices while this block deposits and
+    // flushes, so a block starts with one load latency (its particle fields)
+    // instead of the chain home_list -> block records -> sidx -> fields.
+    const uint32_t G = gridDim.x;
+    uint32_t blk = blockIdx.x;
+    P2GHeader h{};
+    uint32_t pi0 = 0, pi1 = 0, slot1 = 0;
+    if (blk < nh) {
+        p2g_header(a, a.g.home_list[blk], l, h);
+        p2g_first_indices(a, h, pi0, pi1);
+    }
+    if (blk + G < nh) slot1 = a.g.home_list[blk + G];
+    for (; blk < nh; blk += G) {
         for (int t = threadIdx.x; t < kP2GW; t += 256)
             reinterpret_cast<float4*>(lds)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        const uint32_t slot = a.g.home_list[blk];
         uint32_t sid;
         int bx, by, bz;
-        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        unpack_key(h.key, sid, bx, by, bz);
         const SceneDev& sc = a.scenes[sid];
         const float inv_dx = sc.inv_dx;
-        const int l = cx + 4 * cy + 16 * w;
-        const int cbeg = (int)a.g.celloff[(size_t)slot * 64 + l];
-        const int cend = cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + l];
-        const uint32_t off = a.g.blkoff[slot];
-        const int nb = (int)a.g.blkcnt[slot];
+        const int cbeg = h.cbeg, cend = h.cend;
+        const uint32_t off = h.off;
+        const int nb = h.nb;
+        P2GHeader h1{};
+        uint32_t slot2 = 0;
 
         float acc[4][4][kGridCh];
 #pragma unroll
@@ -479,7 +502,7 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
             const int cn = min(kP2GChunk, nb - c0);
             __syncthreads();  // previous chunk fully consumed (and LDS zeroing ordered)
             for (int t = threadIdx.x; t < cn; t += 256) {
-                const uint32_t i = a.bin.sidx[off + c0 + t];
+                const uint32_t i = c0 == 0 ? (t < 256 ? pi0 : pi1) : a.bin.sidx[off + c0 + t];
                 const float g0 = a.cur.x[0][i] * inv_dx, g1 = a.cur.x[1][i] * inv_dx, g2 = a.cur.x[2][i] * inv_dx;
                 const float m = a.cur.m[i];
                 const int q = p2g_swz(t);
@@ -494,6 +517,10 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
                 for (int e = 0; e < 6; ++e) pst[(7 + e) * kP2GStride + q] = a.cur.S[e][i] * inv_dx;
             }
             __syncthreads();
+            if (c0 == 0) {
+                if (blk + G < nh) p2g_header(a, slot1, l, h1);
+                if (blk + 2 * G < nh) slot2 = a.g.home_list[blk + 2 * G];
+            }
             const int kb = max(cbeg - c0, 0), ke = min(cend - c0, cn);
             for (int k = kb; k < ke; ++k) {
                 const int q = p2g_swz(k);
@@ -534,6 +561,7 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
                 }
             }
         }
+        if (blk + G < nh) p2g_first_indices(a, h1, pi0, pi1);
         // Ordered read-add-write deposits: for one (aa, bb) the 64 lanes of the
         // wave touch 64 different nodes; the compiler barrier keeps the
         // program order of the iterations, which the LDS preserves per wave.
@@ -577,6 +605,8 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
             flag(a.st, sid, kStatusOverflow);
         }
         __syncthreads();
+        h = h1;
+        slot1 = slot2;
     }
 }
 
