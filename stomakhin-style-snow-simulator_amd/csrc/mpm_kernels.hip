@@ -355,18 +355,39 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// P2G.  One workgroup per home block, 4 waves.  The block's particles (in
-// sorted order: cell, then particle id) are first staged into LDS by all 256
-// threads with coalesced loads, already reduced to what P2G needs: the offset
-// in the cell f, m, m v and the force matrix scaled by 1/dx.  Then wave w owns
-// the cell layer z = w of the block; lane (cx, cy, d) owns cell (cx, cy, w)
-// and the node plane z = w + d of that cell's 4x4x4 stencil.  A lane walks its
-// cell's particles and keeps the 16 nodes x 7 channels of its plane in
-// registers, so the LDS sees one deposit per cell instead of one per
-// particle.  Deposits go to the wave's private LDS region; for a fixed loop
-// index every lane of a wave hits a different node, and a wave's LDS ops
-// retire in order, so the float sums are deterministic.  The four wave
-// regions are then summed in fixed order into the block's partial sums.
+// P2G on the matrix cores.  One workgroup (4 waves) per home block.  For the
+// particles of one cell the 4x4x4 stencil nodes are fixed, and the reference
+// sums (P2G grid.cpp:119-129, force_grid_accumulation grid.cpp:174-182)
+//   m_i   += w m,   p_i += w m v,
+//   f_i,c += S_c0 dwx wy wz + S_c1 wx dwy wz + S_c2 wx wy dwz   (S = -V0 P F^T)
+// are, for a group of 4 particles of the cell, sums of outer products: with
+// rows (a, b) = the 4x4 x/y nodes and columns (d, ch) = z plane x channel,
+//   C_M[(a,b),(d,ch)] += sum_k (wx_a wy_b)[k] (wz_d q_ch)[k],  q = (m, m v)
+//   C_F[(a,b),(d,c)]  += sum_k (dwx_a wy_b)(wz_d S_c0) + (wx_a dwy_b)(wz_d S_c1)
+//                        + (wx_a wy_b)(dwz_d S_c2)
+// i.e. four v_mfma_f32_16x16x4f32 per group (K = the 4 particles).  Column
+// c = 3 of C_F is unused (its operands are zero).
+// Lane l of a wave supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[4(l/16)+r][l%16], r = 0..3.  Rows are split as (a, b) = (i&3, i>>2) and
+// columns as (d, ch) = (j&3, j>>2), so lane l serves particle k = l/16 of the
+// group with x node a = z plane d = l&3 and y node b = channel ch = (l>>2)&3,
+// and its results are the x nodes r = 0..3 of node (., k, l&3), channel
+// (l>>2)&3: four consecutive x nodes, one b128 in the deposit region.
+//
+// The block's particles (sorted: cell, then particle id) are staged 256 at a
+// time into LDS as 48-float records holding the per-axis B-spline weights and
+// derivatives (grid.cpp:24-58 in grid units), m, m v and S/dx.  A lane
+// fetches its x/z terms with one b128; the six y/channel terms are the same
+// for the 4 lanes of a quad, which fetch them with one b64 each and share
+// them by DPP quad broadcasts.  Wave w takes the 16 cells
+// (w, cy, cz) of the block, cell by cell in sorted order; a group past the
+// end of a cell reads an all-zero record.  After each cell the wave adds its
+// 4x4x4 x 7 tile into its private LDS region (4 x planes x 7 x 7 nodes,
+// channel-major, so a lane's 4 x nodes are one b128) by read-add-write:
+// within one instruction the 64 lanes touch 64 different addresses and a
+// wave's LDS operations complete in order, so every float sum has a fixed
+// order.  The four regions are summed in fixed order into the block's
+// partial sums.
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
 // bit 0, 3 with bit 1.  part stores piece t at node offset piece_base(t)
@@ -379,39 +400,25 @@ __host__ __device__ constexpr int piece_base(int t) {
 }
 static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
 
-constexpr int kP2GNS = 8;                  // floats per node in a deposit region (7 used)
-constexpr int kP2GW = 4 * 49 * kP2GNS;     // one wave region: 4 z-planes x 7 x 7 nodes
-constexpr int kP2GChunk = 512;             // particles staged per pass (>= a typical block)
-constexpr int kP2GStride = kP2GChunk + kP2GChunk / 32;  // per-field stride after the k + k/32 swizzle
-constexpr int kP2GFields = 13;             // fx fy fz m mv0 mv1 mv2 s00 s11 s22 s01 s02 s12
-
-__device__ __forceinline__ int p2g_swz(int k) { return k + (k >> 5); }
-
-// The z weight of stencil plane d (0..3) as a function of the cell offset f:
-// w = x^2 (c3 x + c2) + c0 and dw/df = x (e2 x + e1), with x = 1 - f for the
-// planes d = 0, 2 and x = f for d = 1, 3 -- the pieces of the cubic B-spline
-// (reference grid.cpp:24-58) that bspline() evaluates for all four planes.
-struct ZWeight {
-    bool use_t;
-    float c3, c2, c0, e2, e1;
-};
-__device__ __forceinline__ ZWeight zweight_for(int d) {
-    ZWeight z;
-    z.use_t = (d == 0 || d == 2);
-    const bool outer = (d == 0 || d == 3);
-    z.c3 = outer ? (1.0f / 6.0f) : 0.5f;
-    z.c2 = outer ? 0.0f : -1.0f;
-    z.c0 = outer ? 0.0f : (2.0f / 3.0f);
-    const float sgn = z.use_t ? -1.0f : 1.0f;
-    z.e2 = sgn * (outer ? 0.5f : 1.5f);
-    z.e1 = sgn * (outer ? 0.0f : -2.0f);
-    return z;
-}
+// Deposit region of one wave: [channel 0..7][Y 0..6][Z 0..6][x plane 0..3]
+// (channel 7 is a pad that receives the unused column of C_F).  The channel
+// stride is padded from 196 to 208 floats so that the b128 accesses of 8
+// consecutive lanes (4 z planes x 2 channels) fall into distinct banks.
+constexpr int kP2GStrZ = 4, kP2GStrY = 7 * kP2GStrZ, kP2GStrCh = 208;
+constexpr int kP2GW = 8 * kP2GStrCh;       // one wave region
+constexpr int kP2GChunk = 256;             // particles staged per pass (one per thread)
+constexpr int kP2GRec = 48;                // floats per staged particle record
+// record layout (floats):
+//   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wz_a, dwz_a
+//   [16+8c .. 16+8c+7]   c = 0..3: wy_c, dwy_c, q_c, S_c0, S_c1, S_c2, 0, 0
+//                        (q = m, m v0, m v1, m v2; S_3* = 0)
+constexpr int kP2GRecC = 16;
 
 // Flush table: output o of a home block's partial sums (piece t, channel c,
 // node (X, Y, Z) of the 7^3 region) reads wave regions ww = lo..hi (those
-// whose planes w..w+3 contain Z) at LDS index base + ww * kFlushRegionStep.
-constexpr int kFlushRegionStep = kP2GW - 49 * kP2GNS;
+// whose x planes ww..ww+3 contain X) at LDS index base + ww * kFlushRegionStep,
+// base = c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + X (x plane X - ww).
+constexpr int kFlushRegionStep = kP2GW - 1;
 struct FlushTable {
     uint16_t e[kGridCh * kRegion];
 };
@@ -423,190 +430,225 @@ constexpr FlushTable make_flush_table() {
             for (int idx = 0; idx < sz; ++idx) {
                 const int X = 4 * (pt & 1) + idx % nx, Y = 4 * ((pt >> 1) & 1) + (idx / nx) % ny,
                           Z = 4 * (pt >> 2) + idx / (nx * ny);
-                const int base = ((Z * 7 + Y) * 7 + X) * kP2GNS + c;
-                const int lo = Z > 3 ? Z - 3 : 0, hi = Z < 3 ? Z : 3;
+                const int base = c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + X;
+                const int lo = X > 3 ? X - 3 : 0, hi = X < 3 ? X : 3;
                 t.e[kGridCh * piece_base(pt) + c * sz + idx] = (uint16_t)(base | (lo << 12) | (hi << 14));
             }
     }
     return t;
 }
-static_assert(((6 * 7 + 6) * 7 + 6) * kP2GNS + kGridCh < 4096, "flush table packs the LDS index in 12 bits");
+static_assert(6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 6 < 4096, "flush table packs the LDS index in 12 bits");
+static_assert(kP2GStrCh >= 7 * kP2GStrY, "channel stride holds 7 x 7 x 4 floats");
+static_assert(6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 6 + 3 * kFlushRegionStep < 4 * kP2GW, "flush reads in bounds");
 __constant__ FlushTable kFlush = make_flush_table();
 
-// A home block's records as one P2G lane needs them.
+typedef float p2g_v4f __attribute__((ext_vector_type(4)));
+
+// Value of lane L of this lane's quad (DPP quad_perm [L, L, L, L]).
+template <int L>
+__device__ __forceinline__ float quad_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), L * 0x55, 0xF, 0xF, true));
+}
+
+// A home block's records as the P2G lanes need them: lane n = l & 15 of wave
+// w holds the sorted range of cell (w, n & 3, n >> 2) within the block.
 struct P2GHeader {
     uint64_t key;
     uint32_t off;   // first sorted position
     int nb;         // particles in the block
     int cbeg, cend; // the lane's cell range within the block
 };
-__device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int l, P2GHeader& h) {
+__device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int cell, P2GHeader& h) {
     h.key = a.g.keys[slot];
     h.off = a.g.blkoff[slot];
     h.nb = (int)a.g.blkcnt[slot];
-    h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + l];
-    h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + l];
+    h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + cell];
+    h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + cell];
 }
-// Particle indices of the first staging chunk (sorted positions t and t + 256).
-__device__ __forceinline__ void p2g_first_indices(const StepArgs& a, const P2GHeader& h, uint32_t& i0, uint32_t& i1) {
-    const int t = (int)threadIdx.x, cn = min(kP2GChunk, h.nb);
-    i0 = t < cn ? a.bin.sidx[h.off + t] : 0u;
-    i1 = t + 256 < cn ? a.bin.sidx[h.off + t + 256] : 0u;
+
+// The fields P2G reads of one particle, loaded into registers one chunk
+// ahead of their use so the global-load latency hides under the MFMAs.
+struct P2GRaw {
+    float x[3], v[3], m, S[kStressCh];
+};
+__device__ __forceinline__ void p2g_load(const StepArgs& a, uint32_t i, P2GRaw& r) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        r.x[e] = a.cur.x[e][i];
+        r.v[e] = a.cur.v[e][i];
+    }
+    r.m = a.cur.m[i];
+#pragma unroll
+    for (int e = 0; e < kStressCh; ++e) r.S[e] = a.cur.S[e][i];
+}
+
+// Stage a particle as record rec: weights of the three axes, m, m v, S / dx.
+__device__ __forceinline__ void p2g_stage(const P2GRaw& r, float inv_dx, float* rec) {
+    const float g0 = r.x[0] * inv_dx, g1 = r.x[1] * inv_dx, g2 = r.x[2] * inv_dx;
+    const float m = r.m;
+    const float q1 = m * r.v[0], q2 = m * r.v[1], q3 = m * r.v[2];
+    const float s00 = r.S[0] * inv_dx, s11 = r.S[1] * inv_dx, s22 = r.S[2] * inv_dx;
+    const float s01 = r.S[3] * inv_dx, s02 = r.S[4] * inv_dx, s12 = r.S[5] * inv_dx;
+    float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+    bspline(g0 - floorf(g0), wx, dwx);
+    bspline(g1 - floorf(g1), wy, dwy);
+    bspline(g2 - floorf(g2), wz, dwz);
+    float4* r4 = reinterpret_cast<float4*>(rec);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) r4[e] = make_float4(wx[e], dwx[e], wz[e], dwz[e]);
+    r4[4] = make_float4(wy[0], dwy[0], m, s00);
+    r4[5] = make_float4(s01, s02, 0.0f, 0.0f);
+    r4[6] = make_float4(wy[1], dwy[1], q1, s01);
+    r4[7] = make_float4(s11, s12, 0.0f, 0.0f);
+    r4[8] = make_float4(wy[2], dwy[2], q2, s02);
+    r4[9] = make_float4(s12, s22, 0.0f, 0.0f);
+    r4[10] = make_float4(wy[3], dwy[3], q3, 0.0f);
+    r4[11] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
     __shared__ __align__(16) float lds[4 * kP2GW];
-    __shared__ float pst[kP2GFields * kP2GStride];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int cx = lane & 3, cy = (lane >> 2) & 3, d = lane >> 4;
-    const ZWeight zw = zweight_for(d);
-    const int l = cx + 4 * cy + 16 * w;
+    __shared__ __align__(16) float pst[(kP2GChunk + 1) * kP2GRec];  // + one all-zero record
+    const int t = (int)threadIdx.x;
+    const int lane = t & 63, w = t >> 6;
+    const int kq = lane >> 4, n = lane & 15, na = n & 3, nb = n >> 2;
+    const int my_cell = w + 4 * na + 16 * nb;  // cell (w, cy = na, cz = nb), used for the range loads
+    // this lane's deposit address in its wave region for cell (w, 0, 0): nodes
+    // (X = w + r, Y = cy + kq, Z = cz + na), r = 0..3, channel nb (C_M) or 4 + nb (C_F)
+    const int dep0 = w * kP2GW + nb * kP2GStrCh + kq * kP2GStrY + na * kP2GStrZ;
     const uint32_t nh = a.cnt->n_home;
-    // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
-    // the next block's header is loaded while this block's particles are
-    // accumulated, and its first sorted indices while this block deposits and
-    // flushes, so a block starts with one load latency (its particle fields)
-    // instead of the chain home_list -> block records -> sidx -> fields.
     const uint32_t G = gridDim.x;
     uint32_t blk = blockIdx.x;
-    P2GHeader h{};
-    uint32_t pi0 = 0, pi1 = 0, slot1 = 0;
-    if (blk < nh) {
-        p2g_header(a, a.g.home_list[blk], l, h);
-        p2g_first_indices(a, h, pi0, pi1);
+    if (blk >= nh) return;
+    // The regions are zeroed once here; each flush zeroes what it reads.
+    for (int e = t; e < kP2GW; e += 256) reinterpret_cast<float4*>(lds)[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (t < kP2GRec) pst[kP2GChunk * kP2GRec + t] = 0.0f;
+
+    // Pipeline over this workgroup's chunks (blocks blk, blk + G, ..., each
+    // in chunks of kP2GChunk sorted positions): while chunk k is accumulated,
+    // the fields of chunk k + 1 are loaded into registers and the sorted
+    // indices of chunk k + 2, so no load latency sits between two chunks.
+    // Headers are kept for blocks blk, blk + G and blk + 2G.
+    P2GHeader h{}, hn{}, hnn{};
+    p2g_header(a, a.g.home_list[blk], my_cell, h);
+    if (blk + G < nh) p2g_header(a, a.g.home_list[blk + G], my_cell, hn);
+    if (blk + 2 * G < nh) p2g_header(a, a.g.home_list[blk + 2 * G], my_cell, hnn);
+    // This thread's particle index in chunk c of a block.  The load is issued
+    // unconditionally (from sorted position 0 when the thread has no particle
+    // there) and the flag ok says whether it is used, so no branch or register
+    // reuse forces a wait on the loads in flight.
+    bool ok;
+    auto chunk_index = [&](const P2GHeader& hh, bool exists, int c) -> uint32_t {
+        ok = exists && t < hh.nb - c;
+        return a.bin.sidx[ok ? hh.off + c + t : 0u];
+    };
+    P2GRaw raw{};
+    {
+        const uint32_t i0 = chunk_index(h, true, 0);
+        if (ok) p2g_load(a, i0, raw);
     }
-    if (blk + G < nh) slot1 = a.g.home_list[blk + G];
+    uint32_t inext = kP2GChunk < h.nb ? chunk_index(h, true, kP2GChunk) : chunk_index(hn, blk + G < nh, 0);
+    bool inext_ok = ok;
+
     for (; blk < nh; blk += G) {
-        for (int t = threadIdx.x; t < kP2GW; t += 256)
-            reinterpret_cast<float4*>(lds)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         uint32_t sid;
         int bx, by, bz;
         unpack_key(h.key, sid, bx, by, bz);
-        const SceneDev& sc = a.scenes[sid];
-        const float inv_dx = sc.inv_dx;
-        const int cbeg = h.cbeg, cend = h.cend;
-        const uint32_t off = h.off;
-        const int nb = h.nb;
-        P2GHeader h1{};
-        uint32_t slot2 = 0;
-
-        float acc[4][4][kGridCh];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-#pragma unroll
-                for (int c = 0; c < kGridCh; ++c) acc[i][j][c] = 0.0f;
-
-        for (int c0 = 0; c0 < nb; c0 += kP2GChunk) {
-            const int cn = min(kP2GChunk, nb - c0);
-            __syncthreads();  // previous chunk fully consumed (and LDS zeroing ordered)
-            for (int t = threadIdx.x; t < cn; t += 256) {
-                const uint32_t i = c0 == 0 ? (t < 256 ? pi0 : pi1) : a.bin.sidx[off + c0 + t];
-                const float g0 = a.cur.x[0][i] * inv_dx, g1 = a.cur.x[1][i] * inv_dx, g2 = a.cur.x[2][i] * inv_dx;
-                const float m = a.cur.m[i];
-                const int q = p2g_swz(t);
-                pst[0 * kP2GStride + q] = g0 - floorf(g0);
-                pst[1 * kP2GStride + q] = g1 - floorf(g1);
-                pst[2 * kP2GStride + q] = g2 - floorf(g2);
-                pst[3 * kP2GStride + q] = m;
-                pst[4 * kP2GStride + q] = m * a.cur.v[0][i];
-                pst[5 * kP2GStride + q] = m * a.cur.v[1][i];
-                pst[6 * kP2GStride + q] = m * a.cur.v[2][i];
-#pragma unroll
-                for (int e = 0; e < 6; ++e) pst[(7 + e) * kP2GStride + q] = a.cur.S[e][i] * inv_dx;
-            }
+        const float inv_dx = a.scenes[sid].inv_dx;
+        for (int c0 = 0; c0 < h.nb; c0 += kP2GChunk) {
+            const int cn = min(kP2GChunk, h.nb - c0);
+            __syncthreads();  // pst consumed by every wave (and the flush done)
+            if (t < cn) p2g_stage(raw, inv_dx, pst + t * kP2GRec);
             __syncthreads();
-            if (c0 == 0) {
-                if (blk + G < nh) p2g_header(a, slot1, l, h1);
-                if (blk + 2 * G < nh) slot2 = a.g.home_list[blk + 2 * G];
-            }
-            const int kb = max(cbeg - c0, 0), ke = min(cend - c0, cn);
-            for (int k = kb; k < ke; ++k) {
-                const int q = p2g_swz(k);
-                const float fx = pst[0 * kP2GStride + q], fy = pst[1 * kP2GStride + q], fz = pst[2 * kP2GStride + q];
-                const float m = pst[3 * kP2GStride + q];
-                const float p0 = pst[4 * kP2GStride + q], p1 = pst[5 * kP2GStride + q], p2 = pst[6 * kP2GStride + q];
-                const float s00 = pst[7 * kP2GStride + q], s11 = pst[8 * kP2GStride + q], s22 = pst[9 * kP2GStride + q];
-                const float s01 = pst[10 * kP2GStride + q], s02 = pst[11 * kP2GStride + q],
-                            s12 = pst[12 * kP2GStride + q];
-                float wx[4], dwx[4], wy[4], dwy[4];
-                bspline(fx, wx, dwx);
-                bspline(fy, wy, dwy);
-                const float xz = zw.use_t ? 1.0f - fz : fz;   // this lane's z plane only
-                const float x2 = xz * xz;
-                const float wzd = x2 * (zw.c3 * xz + zw.c2) + zw.c0;
-                const float dwzd = xz * (zw.e2 * xz + zw.e1);
-#pragma unroll
-                for (int bb = 0; bb < 4; ++bb) {
-                    const float yz = wy[bb] * wzd, dyz = dwy[bb] * wzd, ydz = wy[bb] * dwzd;
-                    const float mw = m * yz;
-                    const float q0 = p0 * yz, q1 = p1 * yz, q2 = p2 * yz;
-                    const float B0 = s00 * yz, B1 = s01 * yz, B2 = s02 * yz;
-                    const float A0 = s01 * dyz + s02 * ydz;
-                    const float A1 = s11 * dyz + s12 * ydz;
-                    const float A2 = s12 * dyz + s22 * ydz;
-#pragma unroll
-                    for (int aa = 0; aa < 4; ++aa) {
-                        const float wa = wx[aa], da = dwx[aa];
-                        acc[aa][bb][0] += wa * mw;
-                        acc[aa][bb][1] += wa * q0;
-                        acc[aa][bb][2] += wa * q1;
-                        acc[aa][bb][3] += wa * q2;
-                        // two chained FMAs (acc + (da B + wa A) would cost an extra add)
-                        acc[aa][bb][4] = fmaf(wa, A0, fmaf(da, B0, acc[aa][bb][4]));
-                        acc[aa][bb][5] = fmaf(wa, A1, fmaf(da, B1, acc[aa][bb][5]));
-                        acc[aa][bb][6] = fmaf(wa, A2, fmaf(da, B2, acc[aa][bb][6]));
-                    }
+            // fields of chunk k + 1, then the indices of chunk k + 2
+            if (inext_ok) p2g_load(a, inext, raw);
+            if (c0 + kP2GChunk < h.nb)  // chunk k + 1 is (blk, c0 + chunk)
+                inext = c0 + 2 * kP2GChunk < h.nb ? chunk_index(h, true, c0 + 2 * kP2GChunk)
+                                                  : chunk_index(hn, blk + G < nh, 0);
+            else                        // chunk k + 1 is (blk + G, 0)
+                inext = kP2GChunk < hn.nb ? chunk_index(hn, blk + G < nh, kP2GChunk)
+                                          : chunk_index(hnn, blk + 2 * G < nh, 0);
+            inext_ok = ok;
+            for (int j = 0; j < 16; ++j) {
+                const int lo = max(__builtin_amdgcn_readlane(h.cbeg, j) - c0, 0);
+                const int hi = min(__builtin_amdgcn_readlane(h.cend, j) - c0, cn);
+                if (lo >= hi) continue;
+                // The cell's deposit slots are read first (after the previous
+                // cell's writes, in program order) so their latency hides
+                // under the MFMAs.
+                float4* dm = reinterpret_cast<float4*>(lds + dep0 + (j & 3) * kP2GStrY + (j >> 2) * kP2GStrZ);
+                float4* df = reinterpret_cast<float4*>(reinterpret_cast<float*>(dm) + 4 * kP2GStrCh);
+                float4 tm = *dm, tf = *df;
+                p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
+                // operands of the group at p (the all-zero record past the cell's end)
+                auto fetch = [&](int p, float4& x, float2& c) {
+                    const float* rec = pst + (p + kq < hi ? p + kq : kP2GChunk) * kP2GRec;
+                    x = *reinterpret_cast<const float4*>(rec + 4 * na);  // wx_a dwx_a wz_d dwz_d
+                    c = *reinterpret_cast<const float2*>(rec + kP2GRecC + 8 * nb + 2 * na);
+                };
+                auto group = [&](const float4& rx, const float2& rc) {
+                    const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
+                    const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
+                    const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
+                    const float am = rx.x * wy;  // wx_a wy_b
+                    cm = __builtin_amdgcn_mfma_f32_16x16x4f32(am, rx.z * q, cm, 0, 0, 0);
+                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
+                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
+                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(am, rx.w * s2, cf, 0, 0, 0);
+                };
+                // The first two groups are fetched together (the second is the
+                // zero record for a cell of <= 4 particles); longer cells loop.
+                float4 x0, x1;
+                float2 c0v, c1v;
+                fetch(lo, x0, c0v);
+                fetch(lo + 4, x1, c1v);
+                __builtin_amdgcn_sched_barrier(0);  // issue all reads before the MFMAs
+                group(x0, c0v);
+                group(x1, c1v);
+                for (int p = lo + 8; p < hi; p += 4) {
+                    fetch(p, x0, c0v);
+                    group(x0, c0v);
                 }
+                tm.x += cm[0];
+                tm.y += cm[1];
+                tm.z += cm[2];
+                tm.w += cm[3];
+                tf.x += cf[0];
+                tf.y += cf[1];
+                tf.z += cf[2];
+                tf.w += cf[3];
+                *dm = tm;
+                *df = tf;
+                __asm__ __volatile__("" ::: "memory");  // keep the cells' read-add-writes in program order
             }
         }
-        if (blk + G < nh) p2g_first_indices(a, h1, pi0, pi1);
-        // Ordered read-add-write deposits: for one (aa, bb) the 64 lanes of the
-        // wave touch 64 different nodes; the compiler barrier keeps the
-        // program order of the iterations, which the LDS preserves per wave.
-        float* region = lds + w * kP2GW;
-#pragma unroll
-        for (int aa = 0; aa < 4; ++aa)
-#pragma unroll
-            for (int bb = 0; bb < 4; ++bb) {
-                float* np = region + ((d * 7 + cy + bb) * 7 + cx + aa) * kP2GNS;
-                float4 lo = *reinterpret_cast<float4*>(np);
-                float4 hi = *reinterpret_cast<float4*>(np + 4);
-                lo.x += acc[aa][bb][0];
-                lo.y += acc[aa][bb][1];
-                lo.z += acc[aa][bb][2];
-                lo.w += acc[aa][bb][3];
-                hi.x += acc[aa][bb][4];
-                hi.y += acc[aa][bb][5];
-                hi.z += acc[aa][bb][6];
-                *reinterpret_cast<float4*>(np) = lo;
-                *reinterpret_cast<float4*>(np + 4) = hi;
-                __asm__ __volatile__("" ::: "memory");
-            }
+        const uint32_t slot3 = blk + 3 * G < nh ? a.g.home_list[blk + 3 * G] : 0u;
         __syncthreads();
-
         // Partial sums of the 7x7x7 node region, stored as the eight pieces
         // that fall into the blocks H + t (layout at piece_n) with coalesced
         // writes; each output adds the wave regions holding its node in fixed
-        // order (table kFlush: LDS address and first/last region).
+        // order (table kFlush: LDS address and first/last region) and clears
+        // them for the next block.
         if (blk < a.g.max_home) {
             float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
-            for (int o = threadIdx.x; o < kGridCh * kRegion; o += 256) {
+            for (int o = t; o < kGridCh * kRegion; o += 256) {
                 const uint32_t e = kFlush.e[o];
-                const int base = (int)(e & 0xFFFu), lo = (int)((e >> 12) & 3u), hi = (int)(e >> 14);
+                const int base = (int)(e & 0xFFFu), elo = (int)((e >> 12) & 3u), ehi = (int)(e >> 14);
                 float val = 0.0f;
 #pragma unroll
                 for (int ww = 0; ww < 4; ++ww)
-                    if (ww >= lo && ww <= hi) val += lds[base + ww * kFlushRegionStep];
+                    if (ww >= elo && ww <= ehi) {
+                        val += lds[base + ww * kFlushRegionStep];
+                        lds[base + ww * kFlushRegionStep] = 0.0f;
+                    }
                 out[o] = val;
             }
-        } else if (threadIdx.x == 0) {
+        } else if (t == 0) {
             flag(a.st, sid, kStatusOverflow);
         }
-        __syncthreads();
-        h = h1;
-        slot1 = slot2;
+        h = hn;
+        hn = hnn;
+        if (blk + 3 * G < nh) p2g_header(a, slot3, my_cell, hnn);
     }
 }
 
