@@ -355,9 +355,10 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// P2G on the matrix cores.  One workgroup (4 waves) per home block.  For the
-// particles of one cell the 4x4x4 stencil nodes are fixed, and the reference
-// sums (P2G grid.cpp:119-129, force_grid_accumulation grid.cpp:174-182)
+// P2G on the matrix cores.  One wave per home block, no workgroup barriers.
+// For the particles of one cell the 4x4x4 stencil nodes are fixed, and the
+// reference sums (P2G grid.cpp:119-129, force_grid_accumulation
+// grid.cpp:174-182)
 //   m_i   += w m,   p_i += w m v,
 //   f_i,c += S_c0 dwx wy wz + S_c1 wx dwy wz + S_c2 wx wy dwz   (S = -V0 P F^T)
 // are, for a group of 4 particles of the cell, sums of outer products: with
@@ -366,27 +367,25 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 //   C_F[(a,b),(d,c)]  += sum_k (dwx_a wy_b)(wz_d S_c0) + (wx_a dwy_b)(wz_d S_c1)
 //                        + (wx_a wy_b)(dwz_d S_c2)
 // i.e. four v_mfma_f32_16x16x4f32 per group (K = the 4 particles).  Column
-// c = 3 of C_F is unused (its operands are zero).
-// Lane l of a wave supplies A[l%16][l/16] and B[l/16][l%16] and holds
-// D[4(l/16)+r][l%16], r = 0..3.  Rows are split as (a, b) = (i&3, i>>2) and
-// columns as (d, ch) = (j&3, j>>2), so lane l serves particle k = l/16 of the
-// group with x node a = z plane d = l&3 and y node b = channel ch = (l>>2)&3,
-// and its results are the x nodes r = 0..3 of node (., k, l&3), channel
-// (l>>2)&3: four consecutive x nodes, one b128 in the deposit region.
+// c = 3 of C_F is unused.  Lane l supplies A[l%16][l/16] and B[l/16][l%16]
+// and holds D[4(l/16)+r][l%16], r = 0..3.  Rows split as (a, b) = (i&3, i>>2)
+// and columns as (d, ch) = (j&3, j>>2), so lane l serves particle k = l/16 of
+// the group with x node a = z plane d = l&3 and y node b = channel
+// ch = (l>>2)&3, and holds the x nodes r = 0..3 of node (., k, l&3),
+// channel (l>>2)&3.
 //
-// The block's particles (sorted: cell, then particle id) are staged 256 at a
-// time into LDS as 48-float records holding the per-axis B-spline weights and
-// derivatives (grid.cpp:24-58 in grid units), m, m v and S/dx.  A lane
-// fetches its x/z terms with one b128; the six y/channel terms are the same
-// for the 4 lanes of a quad, which fetch them with one b64 each and share
-// them by DPP quad broadcasts.  Wave w takes the 16 cells
-// (w, cy, cz) of the block, cell by cell in sorted order; a group past the
-// end of a cell reads an all-zero record.  After each cell the wave adds its
-// 4x4x4 x 7 tile into its private LDS region (4 x planes x 7 x 7 nodes,
-// channel-major, so a lane's 4 x nodes are one b128) by read-add-write:
-// within one instruction the 64 lanes touch 64 different addresses and a
-// wave's LDS operations complete in order, so every float sum has a fixed
-// order.  The four regions are summed in fixed order into the block's
+// A wave stages its block's particles (sorted: cell, then particle id) 64 at
+// a time into its own LDS records (per-axis B-spline weights and derivatives
+// of grid.cpp:24-58 in grid units, m, m v, S/dx): a lane fetches its x/z
+// terms with one b128; the six y/channel terms are the same for the 4 lanes
+// of a quad, which fetch them with one b64 each and share them by DPP quad
+// broadcasts.  A group past the end of a cell reads an all-zero record.  The
+// fields of the next chunk are loaded into registers while the current one is
+// accumulated.  The cells of an x row (cx = 0..3 at fixed cy, cz) are
+// consecutive in sorted order; their tiles, shifted by cx, are summed in
+// registers (x nodes 0..7) and added to the wave's LDS region once per x row
+// by read-add-write.  A wave's LDS operations complete in order, so every
+// float sum has a fixed order.  The region is then written as the block's
 // partial sums.
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
@@ -400,25 +399,27 @@ __host__ __device__ constexpr int piece_base(int t) {
 }
 static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
 
-// Deposit region of one wave: [channel 0..7][Y 0..6][Z 0..6][x plane 0..3]
-// (channel 7 is a pad that receives the unused column of C_F).  The channel
-// stride is padded from 196 to 208 floats so that the b128 accesses of 8
-// consecutive lanes (4 z planes x 2 channels) fall into distinct banks.
-constexpr int kP2GStrZ = 4, kP2GStrY = 7 * kP2GStrZ, kP2GStrCh = 208;
-constexpr int kP2GW = 8 * kP2GStrCh;       // one wave region
-constexpr int kP2GChunk = 256;             // particles staged per pass (one per thread)
-constexpr int kP2GRec = 48;                // floats per staged particle record
+// Region of a wave: [x half 0..1][channel 0..6][Y 0..6][Z 0..6][x 0..3] (x half
+// h holds x nodes 4h..4h+3).  The channel stride is padded from 196 to 208
+// floats so the b128 accesses of 8 consecutive lanes (4 z planes x 2
+// channels) fall into distinct banks.
+constexpr int kP2GStrZ = 4, kP2GStrY = 7 * kP2GStrZ, kP2GStrCh = 208, kP2GStrH = kGridCh * kP2GStrCh;
+constexpr int kP2GReg = 2 * kP2GStrH;      // floats per wave region
+constexpr int kP2GChunk = 64;              // particles staged per pass (one per lane)
+constexpr int kP2GRec = 40;                // floats per staged particle record
 // record layout (floats):
 //   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wz_a, dwz_a
-//   [16+8c .. 16+8c+7]   c = 0..3: wy_c, dwy_c, q_c, S_c0, S_c1, S_c2, 0, 0
+//   [16+6c .. 16+6c+5]   c = 0..3: wy_c, dwy_c, q_c, S_c0, S_c1, S_c2
 //                        (q = m, m v0, m v1, m v2; S_3* = 0)
+// Lane (quad member) n reads the pair [16+6c+2n, +1]; member 3's pair is
+// never used (it overlaps the next record, or the 2 floats of slack).
 constexpr int kP2GRecC = 16;
+constexpr int kP2GPst = (kP2GChunk + 1) * kP2GRec + 2;  // + the all-zero record + slack
+constexpr int kP2GFlushQ = (((kGridCh * kRegion + 63) / 64) + 1) & ~1;  // flush outputs per lane (even)
+static_assert(kP2GStrCh >= 7 * kP2GStrY, "channel stride holds 7 x 7 x 4 floats");
 
 // Flush table: output o of a home block's partial sums (piece t, channel c,
-// node (X, Y, Z) of the 7^3 region) reads wave regions ww = lo..hi (those
-// whose x planes ww..ww+3 contain X) at LDS index base + ww * kFlushRegionStep,
-// base = c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + X (x plane X - ww).
-constexpr int kFlushRegionStep = kP2GW - 1;
+// node (X, Y, Z) of the 7^3 region) is region index e[o].
 struct FlushTable {
     uint16_t e[kGridCh * kRegion];
 };
@@ -430,16 +431,13 @@ constexpr FlushTable make_flush_table() {
             for (int idx = 0; idx < sz; ++idx) {
                 const int X = 4 * (pt & 1) + idx % nx, Y = 4 * ((pt >> 1) & 1) + (idx / nx) % ny,
                           Z = 4 * (pt >> 2) + idx / (nx * ny);
-                const int base = c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + X;
-                const int lo = X > 3 ? X - 3 : 0, hi = X < 3 ? X : 3;
-                t.e[kGridCh * piece_base(pt) + c * sz + idx] = (uint16_t)(base | (lo << 12) | (hi << 14));
+                t.e[kGridCh * piece_base(pt) + c * sz + idx] =
+                    (uint16_t)((X >> 2) * kP2GStrH + c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + (X & 3));
             }
     }
     return t;
 }
-static_assert(6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 6 < 4096, "flush table packs the LDS index in 12 bits");
-static_assert(kP2GStrCh >= 7 * kP2GStrY, "channel stride holds 7 x 7 x 4 floats");
-static_assert(6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 6 + 3 * kFlushRegionStep < 4 * kP2GW, "flush reads in bounds");
+static_assert(kP2GStrH + 6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 2 < kP2GReg, "flush reads in bounds");
 __constant__ FlushTable kFlush = make_flush_table();
 
 typedef float p2g_v4f __attribute__((ext_vector_type(4)));
@@ -450,20 +448,20 @@ __device__ __forceinline__ float quad_bcast(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), L * 0x55, 0xF, 0xF, true));
 }
 
-// A home block's records as the P2G lanes need them: lane n = l & 15 of wave
-// w holds the sorted range of cell (w, n & 3, n >> 2) within the block.
+// A home block's records as a P2G wave needs them: lane l holds cell l's
+// sorted range within the block.
 struct P2GHeader {
     uint64_t key;
     uint32_t off;   // first sorted position
     int nb;         // particles in the block
-    int cbeg, cend; // the lane's cell range within the block
+    int cbeg, cend; // cell (lane) range within the block
 };
-__device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int cell, P2GHeader& h) {
+__device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int lane, P2GHeader& h) {
     h.key = a.g.keys[slot];
     h.off = a.g.blkoff[slot];
     h.nb = (int)a.g.blkcnt[slot];
-    h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + cell];
-    h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + cell];
+    h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + lane];
+    h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + lane];
 }
 
 // The fields P2G reads of one particle, loaded into registers one chunk
@@ -496,51 +494,71 @@ __device__ __forceinline__ void p2g_stage(const P2GRaw& r, float inv_dx, float* 
     float4* r4 = reinterpret_cast<float4*>(rec);
 #pragma unroll
     for (int e = 0; e < 4; ++e) r4[e] = make_float4(wx[e], dwx[e], wz[e], dwz[e]);
-    r4[4] = make_float4(wy[0], dwy[0], m, s00);
-    r4[5] = make_float4(s01, s02, 0.0f, 0.0f);
-    r4[6] = make_float4(wy[1], dwy[1], q1, s01);
-    r4[7] = make_float4(s11, s12, 0.0f, 0.0f);
-    r4[8] = make_float4(wy[2], dwy[2], q2, s02);
-    r4[9] = make_float4(s12, s22, 0.0f, 0.0f);
-    r4[10] = make_float4(wy[3], dwy[3], q3, 0.0f);
-    r4[11] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float2* r2 = reinterpret_cast<float2*>(rec + kP2GRecC);
+    r2[0] = make_float2(wy[0], dwy[0]);
+    r2[1] = make_float2(m, s00);
+    r2[2] = make_float2(s01, s02);
+    r2[3] = make_float2(wy[1], dwy[1]);
+    r2[4] = make_float2(q1, s01);
+    r2[5] = make_float2(s11, s12);
+    r2[6] = make_float2(wy[2], dwy[2]);
+    r2[7] = make_float2(q2, s02);
+    r2[8] = make_float2(s12, s22);
+    r2[9] = make_float2(wy[3], dwy[3]);
+    r2[10] = make_float2(q3, 0.0f);
+    r2[11] = make_float2(0.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
-    __shared__ __align__(16) float lds[4 * kP2GW];
-    __shared__ __align__(16) float pst[(kP2GChunk + 1) * kP2GRec];  // + one all-zero record
-    const int t = (int)threadIdx.x;
-    const int lane = t & 63, w = t >> 6;
+// Adds lane values t[0..3] to the b128 at p (read-add-write).
+__device__ __forceinline__ void p2g_add4(float* p, float t0, float t1, float t2, float t3) {
+    float4 v = *reinterpret_cast<float4*>(p);
+    v.x += t0;
+    v.y += t1;
+    v.z += t2;
+    v.w += t3;
+    *reinterpret_cast<float4*>(p) = v;
+}
+
+__global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
+    __shared__ __align__(16) float reg[kP2GReg];
+    __shared__ __align__(16) float pst[kP2GPst];
+    const int lane = (int)threadIdx.x;
     const int kq = lane >> 4, n = lane & 15, na = n & 3, nb = n >> 2;
-    const int my_cell = w + 4 * na + 16 * nb;  // cell (w, cy = na, cz = nb), used for the range loads
-    // this lane's deposit address in its wave region for cell (w, 0, 0): nodes
-    // (X = w + r, Y = cy + kq, Z = cz + na), r = 0..3, channel nb (C_M) or 4 + nb (C_F)
-    const int dep0 = w * kP2GW + nb * kP2GStrCh + kq * kP2GStrY + na * kP2GStrZ;
+    // this lane's deposit address for x row (cy, cz) = (0, 0): nodes
+    // (X, Y = cy + kq, Z = cz + na), channel nb (C_M) or 4 + nb (C_F)
+    const int dep0 = nb * kP2GStrCh + kq * kP2GStrY + na * kP2GStrZ;
     const uint32_t nh = a.cnt->n_home;
     const uint32_t G = gridDim.x;
     uint32_t blk = blockIdx.x;
     if (blk >= nh) return;
-    // The regions are zeroed once here; each flush zeroes what it reads.
-    for (int e = t; e < kP2GW; e += 256) reinterpret_cast<float4*>(lds)[e] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (t < kP2GRec) pst[kP2GChunk * kP2GRec + t] = 0.0f;
+    for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < kP2GRec + 2) pst[kP2GChunk * kP2GRec + lane] = 0.0f;
+    // This lane's flush outputs o = lane + 64 q and their region indices,
+    // two 16-bit indices per register, loaded once.
+    uint32_t ftab[kP2GFlushQ / 2];
+#pragma unroll
+    for (int q = 0; q < kP2GFlushQ; q += 2) {
+        const int o0 = lane + 64 * q, o1 = o0 + 64;
+        const uint32_t e0 = o0 < kGridCh * kRegion ? kFlush.e[o0] : 0u;
+        const uint32_t e1 = o1 < kGridCh * kRegion ? kFlush.e[o1] : 0u;
+        ftab[q / 2] = e0 | (e1 << 16);
+    }
 
-    // Pipeline over this workgroup's chunks (blocks blk, blk + G, ..., each
-    // in chunks of kP2GChunk sorted positions): while chunk k is accumulated,
-    // the fields of chunk k + 1 are loaded into registers and the sorted
-    // indices of chunk k + 2, so no load latency sits between two chunks.
-    // Headers are kept for blocks blk, blk + G and blk + 2G.
+    // Chunk pipeline over this wave's blocks blk, blk + G, ...: while chunk k
+    // is accumulated, the fields of chunk k + 1 and the sorted indices of
+    // chunk k + 2 are loaded.  Headers are kept for blocks blk, blk + G, blk + 2G.
     P2GHeader h{}, hn{}, hnn{};
-    p2g_header(a, a.g.home_list[blk], my_cell, h);
-    if (blk + G < nh) p2g_header(a, a.g.home_list[blk + G], my_cell, hn);
-    if (blk + 2 * G < nh) p2g_header(a, a.g.home_list[blk + 2 * G], my_cell, hnn);
-    // This thread's particle index in chunk c of a block.  The load is issued
-    // unconditionally (from sorted position 0 when the thread has no particle
-    // there) and the flag ok says whether it is used, so no branch or register
-    // reuse forces a wait on the loads in flight.
+    p2g_header(a, a.g.home_list[blk], lane, h);
+    if (blk + G < nh) p2g_header(a, a.g.home_list[blk + G], lane, hn);
+    if (blk + 2 * G < nh) p2g_header(a, a.g.home_list[blk + 2 * G], lane, hnn);
+    // This lane's particle index in chunk c of a block.  The load is issued
+    // unconditionally (sorted position 0 when the lane has no particle there)
+    // and the flag says whether it is used, so no branch forces a wait on the
+    // loads in flight.
     bool ok;
     auto chunk_index = [&](const P2GHeader& hh, bool exists, int c) -> uint32_t {
-        ok = exists && t < hh.nb - c;
-        return a.bin.sidx[ok ? hh.off + c + t : 0u];
+        ok = exists && lane < hh.nb - c;
+        return a.bin.sidx[ok ? hh.off + c + lane : 0u];
     };
     P2GRaw raw{};
     {
@@ -550,16 +568,43 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
     uint32_t inext = kP2GChunk < h.nb ? chunk_index(h, true, kP2GChunk) : chunk_index(hn, blk + G < nh, 0);
     bool inext_ok = ok;
 
+    // operands of the group at chunk position p of a cell ending at hi (the
+    // all-zero record for the lanes past the end)
+    const float* lane_x = pst + kq * kP2GRec + 4 * na;
+    const float* lane_c = pst + kq * kP2GRec + kP2GRecC + 6 * nb + 2 * na;
+    const int zero_d = (kP2GChunk - kq) * kP2GRec;  // from record kq to the zero record
+    auto fetch = [&](int p, int hi, float4& x, float2& c) {
+        const int d = p * kP2GRec + (p + kq < hi ? 0 : zero_d - p * kP2GRec);
+        x = *reinterpret_cast<const float4*>(lane_x + d);  // wx_a dwx_a wz_d dwz_d
+        c = *reinterpret_cast<const float2*>(lane_c + d);
+    };
+
     for (; blk < nh; blk += G) {
         uint32_t sid;
         int bx, by, bz;
         unpack_key(h.key, sid, bx, by, bz);
         const float inv_dx = a.scenes[sid].inv_dx;
+        // x-row accumulators: C_M and C_F of x nodes 0..7 of the current x row
+        float am[8], af[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) am[e] = af[e] = 0.0f;
+        int row = 0;  // current x row (cy + 4 cz)
+        int l = 0;    // next cell to visit
+        auto deposit_row = [&]() {
+            float* d = reg + dep0 + (row & 3) * kP2GStrY + (row >> 2) * kP2GStrZ;
+            p2g_add4(d, am[0], am[1], am[2], am[3]);
+            p2g_add4(d + kP2GStrH, am[4], am[5], am[6], am[7]);
+            if (nb < 3) {
+                p2g_add4(d + 4 * kP2GStrCh, af[0], af[1], af[2], af[3]);
+                p2g_add4(d + 4 * kP2GStrCh + kP2GStrH, af[4], af[5], af[6], af[7]);
+            }
+            __asm__ __volatile__("" ::: "memory");  // keep the rows' read-add-writes in program order
+#pragma unroll
+            for (int e = 0; e < 8; ++e) am[e] = af[e] = 0.0f;
+        };
         for (int c0 = 0; c0 < h.nb; c0 += kP2GChunk) {
             const int cn = min(kP2GChunk, h.nb - c0);
-            __syncthreads();  // pst consumed by every wave (and the flush done)
-            if (t < cn) p2g_stage(raw, inv_dx, pst + t * kP2GRec);
-            __syncthreads();
+            if (lane < cn) p2g_stage(raw, inv_dx, pst + lane * kP2GRec);
             // fields of chunk k + 1, then the indices of chunk k + 2
             if (inext_ok) p2g_load(a, inext, raw);
             if (c0 + kP2GChunk < h.nb)  // chunk k + 1 is (blk, c0 + chunk)
@@ -569,86 +614,86 @@ __global__ __launch_bounds__(256) void k_p2g(StepArgs a) {
                 inext = kP2GChunk < hn.nb ? chunk_index(hn, blk + G < nh, kP2GChunk)
                                           : chunk_index(hnn, blk + 2 * G < nh, 0);
             inext_ok = ok;
-            for (int j = 0; j < 16; ++j) {
-                const int lo = max(__builtin_amdgcn_readlane(h.cbeg, j) - c0, 0);
-                const int hi = min(__builtin_amdgcn_readlane(h.cend, j) - c0, cn);
-                if (lo >= hi) continue;
-                // The cell's deposit slots are read first (after the previous
-                // cell's writes, in program order) so their latency hides
-                // under the MFMAs.
-                float4* dm = reinterpret_cast<float4*>(lds + dep0 + (j & 3) * kP2GStrY + (j >> 2) * kP2GStrZ);
-                float4* df = reinterpret_cast<float4*>(reinterpret_cast<float*>(dm) + 4 * kP2GStrCh);
-                float4 tm = *dm, tf = *df;
-                p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
-                // operands of the group at p (the all-zero record past the cell's end)
-                auto fetch = [&](int p, float4& x, float2& c) {
-                    const float* rec = pst + (p + kq < hi ? p + kq : kP2GChunk) * kP2GRec;
-                    x = *reinterpret_cast<const float4*>(rec + 4 * na);  // wx_a dwx_a wz_d dwz_d
-                    c = *reinterpret_cast<const float2*>(rec + kP2GRecC + 8 * nb + 2 * na);
-                };
-                auto group = [&](const float4& rx, const float2& rc) {
-                    const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
-                    const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
-                    const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
-                    const float am = rx.x * wy;  // wx_a wy_b
-                    cm = __builtin_amdgcn_mfma_f32_16x16x4f32(am, rx.z * q, cm, 0, 0, 0);
-                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
-                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
-                    cf = __builtin_amdgcn_mfma_f32_16x16x4f32(am, rx.w * s2, cf, 0, 0, 0);
-                };
-                // The first two groups are fetched together (the second is the
-                // zero record for a cell of <= 4 particles); longer cells loop.
-                float4 x0, x1;
-                float2 c0v, c1v;
-                fetch(lo, x0, c0v);
-                fetch(lo + 4, x1, c1v);
-                __builtin_amdgcn_sched_barrier(0);  // issue all reads before the MFMAs
-                group(x0, c0v);
-                group(x1, c1v);
-                for (int p = lo + 8; p < hi; p += 4) {
-                    fetch(p, x0, c0v);
-                    group(x0, c0v);
+
+            // cells l.. that reach into this chunk, in sorted order
+            for (; l < 64; ++l) {
+                const int cb = __builtin_amdgcn_readlane(h.cbeg, l) - c0;
+                const int ce = __builtin_amdgcn_readlane(h.cend, l) - c0;
+                if (cb >= cn) break;  // starts in a later chunk
+                const int lo = max(cb, 0), hi = min(ce, cn);
+                if (lo < hi) {
+                    if ((l >> 2) != row) {
+                        deposit_row();
+                        row = l >> 2;
+                    }
+                    float4 x0, x1;
+                    float2 q0, q1;
+                    fetch(lo, hi, x0, q0);
+                    fetch(lo + 4, hi, x1, q1);
+                    p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
+                    auto group = [&](const float4& rx, const float2& rc) {
+                        const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
+                        const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
+                        const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
+                        const float wxy = rx.x * wy;  // wx_a wy_b
+                        cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
+                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
+                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
+                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
+                    };
+                    // the first two groups (the second is the zero record for a
+                    // cell piece of <= 4 particles); longer pieces loop
+                    group(x0, q0);
+                    group(x1, q1);
+                    for (int p = lo + 8; p < hi; p += 4) {
+                        fetch(p, hi, x0, q0);
+                        group(x0, q0);
+                    }
+                    // x nodes cx + r of the row
+                    switch (l & 3) {
+                        case 0:
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r] += cm[r], af[r] += cf[r];
+                            break;
+                        case 1:
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r + 1] += cm[r], af[r + 1] += cf[r];
+                            break;
+                        case 2:
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r + 2] += cm[r], af[r + 2] += cf[r];
+                            break;
+                        default:
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r + 3] += cm[r], af[r + 3] += cf[r];
+                            break;
+                    }
                 }
-                tm.x += cm[0];
-                tm.y += cm[1];
-                tm.z += cm[2];
-                tm.w += cm[3];
-                tf.x += cf[0];
-                tf.y += cf[1];
-                tf.z += cf[2];
-                tf.w += cf[3];
-                *dm = tm;
-                *df = tf;
-                __asm__ __volatile__("" ::: "memory");  // keep the cells' read-add-writes in program order
+                if (ce > cn) break;  // continues in the next chunk
             }
         }
-        const uint32_t slot3 = blk + 3 * G < nh ? a.g.home_list[blk + 3 * G] : 0u;
-        __syncthreads();
+        deposit_row();
+
         // Partial sums of the 7x7x7 node region, stored as the eight pieces
         // that fall into the blocks H + t (layout at piece_n) with coalesced
-        // writes; each output adds the wave regions holding its node in fixed
-        // order (table kFlush: LDS address and first/last region) and clears
-        // them for the next block.
+        // writes (table kFlush: region index of each output), clearing the
+        // region for the next block.
+        const uint32_t slot3 = blk + 3 * G < nh ? a.g.home_list[blk + 3 * G] : 0u;
         if (blk < a.g.max_home) {
             float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
-            for (int o = t; o < kGridCh * kRegion; o += 256) {
-                const uint32_t e = kFlush.e[o];
-                const int base = (int)(e & 0xFFFu), elo = (int)((e >> 12) & 3u), ehi = (int)(e >> 14);
-                float val = 0.0f;
 #pragma unroll
-                for (int ww = 0; ww < 4; ++ww)
-                    if (ww >= elo && ww <= ehi) {
-                        val += lds[base + ww * kFlushRegionStep];
-                        lds[base + ww * kFlushRegionStep] = 0.0f;
-                    }
-                out[o] = val;
+            for (int q = 0; q < kP2GFlushQ; ++q) {
+                const int o = lane + 64 * q;
+                if (o < kGridCh * kRegion) out[o] = reg[(ftab[q / 2] >> (16 * (q & 1))) & 0xFFFFu];
             }
-        } else if (t == 0) {
+        } else if (lane == 0) {
             flag(a.st, sid, kStatusOverflow);
         }
+        __asm__ __volatile__("" ::: "memory");  // region reads before the clearing writes
+        for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         h = hn;
         hn = hnn;
-        if (blk + 3 * G < nh) p2g_header(a, slot3, my_cell, hnn);
+        if (blk + 3 * G < nh) p2g_header(a, slot3, lane, hnn);
     }
 }
 
@@ -1403,7 +1448,7 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[1], s);
-    k_p2g<<<c.grid_w, 256, 0, s>>>(a);
+    k_p2g<<<c.grid_w * 4, 64, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[2], s);
     k_grid<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
@@ -1432,7 +1477,7 @@ void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_list<<<c.grid_l, 1024, 0, s>>>(a, 1);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
-    k_p2g<<<c.grid_w, 256, 0, s>>>(a);
+    k_p2g<<<c.grid_w * 4, 64, 0, s>>>(a);
     StepArgs am = a;
     am.mass_only = 1;
     k_grid<<<c.grid_b, 256, 0, s>>>(am);
