@@ -574,7 +574,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     const float* lane_c = pst + kq * kP2GRec + kP2GRecC + 6 * nb + 2 * na;
     const int zero_d = (kP2GChunk - kq) * kP2GRec;  // from record kq to the zero record
     auto fetch = [&](int p, int hi, float4& x, float2& c) {
-        const int d = p * kP2GRec + (p + kq < hi ? 0 : zero_d - p * kP2GRec);
+        const int d = p + kq < hi ? p * kP2GRec : zero_d;
         x = *reinterpret_cast<const float4*>(lane_x + d);  // wx_a dwx_a wz_d dwz_d
         c = *reinterpret_cast<const float2*>(lane_c + d);
     };
@@ -630,6 +630,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                     float2 q0, q1;
                     fetch(lo, hi, x0, q0);
                     fetch(lo + 4, hi, x1, q1);
+                    __builtin_amdgcn_sched_barrier(0);  // both groups' reads in flight before the MFMAs
                     p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
                     auto group = [&](const float4& rx, const float2& rc) {
                         const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
@@ -1099,6 +1100,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 ly = min(max(ly, 0), 3);
                 lz = min(max(lz, 0), 3);
             }
+            // The deformation state is loaded before the gather so its latency
+            // hides under the gather.
+            float Fe[9];
+#pragma unroll
+            for (int e = 0; e < 9; ++e) Fe[e] = a.cur.F[e][i];
+            const float Jp = a.cur.Jp[i];
+            const float V0 = a.cur.V0[i];
+            float mu0, la0;
+            lame(sc, a.cur, i, mu0, la0);
             float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
             bspline(f[0], wx, dwx);
             bspline(f[1], wy, dwy);
@@ -1148,13 +1158,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 #pragma unroll
             for (int e = 0; e < 9; ++e) L[e] *= inv_dx;
 
-            float Fe[9];
-#pragma unroll
-            for (int e = 0; e < 9; ++e) Fe[e] = a.cur.F[e][i];
-            const float Jp = a.cur.Jp[i];
-            const float V0 = a.cur.V0[i];
-            float mu0, la0;
-            lame(sc, a.cur, i, mu0, la0);
             float Fp_old[9], Fp_new[9];
             constexpr bool track = TRACK;
 #pragma unroll
