@@ -102,16 +102,32 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); wbase < n; wbase += stride) {
+    // The scene id and position of the next pass are loaded while this pass
+    // waits on its hash lookup and cell-count atomics.
+    uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+    uint32_t nsid = 0;
+    float nx0 = 0.f, nx1 = 0.f, nx2 = 0.f;
+    if (wbase + lane < n) {
+        nsid = a.cur.sid[wbase + lane];
+        nx0 = a.cur.x[0][wbase + lane];
+        nx1 = a.cur.x[1][wbase + lane];
+        nx2 = a.cur.x[2][wbase + lane];
+    }
+    for (; wbase < n; wbase += stride) {
         const uint32_t i = wbase + lane;
+        const float x0 = nx0, x1 = nx1, x2 = nx2;
         uint64_t key = kEmptyKey;
-        uint32_t l = 0, sid = 0;
+        uint32_t l = 0, sid = nsid;
         bool want = false;
+        if (wbase + stride + lane < n) {
+            nsid = a.cur.sid[wbase + stride + lane];
+            nx0 = a.cur.x[0][wbase + stride + lane];
+            nx1 = a.cur.x[1][wbase + stride + lane];
+            nx2 = a.cur.x[2][wbase + stride + lane];
+        }
         if (i < n) {
-            sid = a.cur.sid[i];
             const SceneDev& sc = a.scenes[sid];
             if (sc.flags & a.pass_flag) {
-                const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
                 if (!finite3(x0, x1, x2)) {
                     flag(a.st, sid, kStatusNaN);
                 } else {
@@ -283,16 +299,33 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
     }
 }
 
+// Counting-sort scatter.  Each thread handles 4 particles per pass with all
+// their loads issued together, so the dependent chain key -> block/cell
+// offsets -> store overlaps across them.
 __global__ __launch_bounds__(256) void k_scatter(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const uint32_t key32 = a.bin.pkey[i];
-        if (key32 == kInvalid) continue;
-        const uint32_t slot = key32 >> 6, l = key32 & 63;
-        const uint32_t dst = a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + l] + a.bin.prank[i];
-        a.bin.sidx[dst] = i;
-        a.bin.spid[dst] = a.cur.pid[i];
+    for (uint32_t i0 = blockIdx.x * blockDim.x + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        uint32_t key[4], rank[4], pid[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t i = i0 + k * stride;
+            key[k] = i < n ? a.bin.pkey[i] : kInvalid;
+            rank[k] = i < n ? a.bin.prank[i] : 0u;
+            pid[k] = i < n ? a.cur.pid[i] : 0u;
+        }
+        uint32_t dst[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t slot = key[k] >> 6, l = key[k] & 63;
+            dst[k] = key[k] == kInvalid ? 0u : a.g.blkoff[slot] + a.g.celloff[(size_t)slot * 64 + l] + rank[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (key[k] != kInvalid) {
+                a.bin.sidx[dst[k]] = i0 + k * stride;
+                a.bin.spid[dst[k]] = pid[k];
+            }
     }
 }
 

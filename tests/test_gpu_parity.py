@@ -65,6 +65,30 @@ def test_ten_steps_match_oracle(kind, scale):
     assert np.abs(got["Fe"] - ref["Fe"]).max() < 1e-4
 
 
+@pytest.mark.parametrize("seed,n,side", [(5, 3000, 0.1), (6, 900, 0.05)])
+def test_random_density_matches_oracle(seed, n, side):
+    """Uniformly random positions, so cells hold anywhere from 0 to ~50
+    particles: P2G's cells of more than 8 particles, cells split across its
+    64-particle staging chunks, empty cells, partial x rows and blocks of many
+    chunks are all exercised (a jittered 8-per-cell lattice hits none of
+    them).  Setup + 10 steps against the oracle."""
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0.0, side, size=(n, 3)) + np.array([0.31, 0.047, -0.23])
+    v = rng.normal(0.0, 0.5, size=(n, 3))
+    prm = snowmpm.params(1e-4, 0.02, *snowmpm.lame(1.4e5, 0.2), 10.0, 0.025, 0.0075, 0.95)
+    sc = dict(x=x, v=v, m=np.full(n, 4e-4), params=prm, colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
+    cells = np.floor(x / prm.dx).astype(int)
+    _, counts = np.unique(cells, axis=0, return_counts=True)
+    assert counts.max() > 16, counts.max()  # the long-cell path is reached
+    ref = mpmref.run_oracle(sc, 10)
+    got = mpmref.run_gpu(sc, 10)
+    ex, ev = mpmref.rel_errors(got, ref, sc)
+    print(f"random density n={n}, max {counts.max()} per cell: rel err x {ex:.3e} v {ev:.3e}")
+    assert ex < TOL and ev < TOL
+    assert np.abs(got["V0"] - ref["V0"]).max() / ref["V0"].max() < 1e-5
+    assert got["status"][0] == 0
+
+
 def test_contact_plasticity_matches_oracle():
     """Cube hitting the ground fast enough to plastify: F_E, J_P and v vs oracle."""
     sc = _scene(0, 0.3)
