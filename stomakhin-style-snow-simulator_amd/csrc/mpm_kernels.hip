@@ -1066,12 +1066,21 @@ __device__ __forceinline__ void g2p_stage_load(const StepArgs& a, uint32_t ts, i
     for (int i = 0; i < kG2PStage; ++i) r[i] = src[l32 + 32 * i];
 }
 
-__device__ __forceinline__ void g2p_stage_store(float* vu, int tg, int l32, const float (&r)[kG2PStage]) {
+// Staged node layout: node n = Z * 64 + Y * 8 + X of the 8x8x8 nodes has
+// (v0, v1, v2, u0) at va[n] and (u1, u2) at ub[n], so a gather reads one b128
+// and one b64 per node.
+__device__ __forceinline__ void g2p_stage_store(float4* va, float2* ub, int tg, int l32, const float (&r)[kG2PStage]) {
+    float* fa = reinterpret_cast<float*>(va);
+    float* fb = reinterpret_cast<float*>(ub);
 #pragma unroll
     for (int i = 0; i < kG2PStage; ++i) {
         const int e = l32 + 32 * i, c = e >> 6, nn = e & 63;
         const int X = (tg & 1) * 4 + (nn & 3), Y = ((tg >> 1) & 1) * 4 + ((nn >> 2) & 3), Z = (tg >> 2) * 4 + (nn >> 4);
-        vu[(Z * 64 + Y * 8 + X) * 8 + c + (c >= 3 ? 1 : 0)] = r[i];
+        const int node = Z * 64 + Y * 8 + X;
+        if (c < 4)
+            fa[node * 4 + c] = r[i];
+        else
+            fb[node * 2 + c - 4] = r[i];
     }
 }
 
@@ -1083,7 +1092,8 @@ __device__ __forceinline__ void g2p_stage_store(float* vu, int tg, int l32, cons
 // spills and the extra resident waves hide the particle and grid loads.
 template <bool TRACK, bool PERMUTE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
-    __shared__ __align__(16) float vu[512 * 8];  // [Z][Y][X][v0 v1 v2 - u0 u1 u2 -]
+    __shared__ float4 va[512];  // [Z][Y][X]: v0 v1 v2 u0
+    __shared__ float2 ub[512];  // [Z][Y][X]: u1 u2
     const uint32_t nh = a.cnt->n_home;
     // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
     // while block blk is computed, the grid data of blk + G is in flight into
@@ -1104,7 +1114,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         ts1 = a.g.nb[(size_t)slot1 * 8 + tg];
     }
     for (; blk < nh; blk += G) {
-        g2p_stage_store(vu, tg, l32, stg);
+        g2p_stage_store(va, ub, tg, l32, stg);
         uint32_t sid;
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
@@ -1160,9 +1170,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                     float S0 = 0.f, S1 = 0.f, S2 = 0.f, X0 = 0.f, X1 = 0.f, X2 = 0.f, U0 = 0.f, U1 = 0.f, U2 = 0.f;
 #pragma unroll
                     for (int aa = 0; aa < 4; ++aa) {
-                        const float4 gv = *reinterpret_cast<const float4*>(&vu[(rowb + aa) * 8]);
-                        const float4 gu = *reinterpret_cast<const float4*>(&vu[(rowb + aa) * 8 + 4]);
-                        const float gv0 = gv.x, gv1 = gv.y, gv2 = gv.z, gu0 = gu.x, gu1 = gu.y, gu2 = gu.z;
+                        const float4 gv = va[rowb + aa];
+                        const float2 gu = ub[rowb + aa];
+                        const float gv0 = gv.x, gv1 = gv.y, gv2 = gv.z, gu0 = gv.w, gu1 = gu.x, gu2 = gu.y;
                         S0 += wx[aa] * gv0;
                         S1 += wx[aa] * gv1;
                         S2 += wx[aa] * gv2;
