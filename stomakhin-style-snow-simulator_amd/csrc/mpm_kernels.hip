@@ -438,7 +438,8 @@ static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
 // channels) fall into distinct banks.
 constexpr int kP2GStrZ = 4, kP2GStrY = 7 * kP2GStrZ, kP2GStrCh = 208, kP2GStrH = kGridCh * kP2GStrCh;
 constexpr int kP2GReg = 2 * kP2GStrH;      // floats per wave region
-constexpr int kP2GChunk = 64;              // particles staged per pass (one per lane)
+constexpr int kP2GChunk = 32;              // particles staged per pass (at most one per lane)
+static_assert(kP2GChunk <= 64, "one staged particle per lane");
 constexpr int kP2GRec = 40;                // floats per staged particle record
 // record layout (floats):
 //   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wz_a, dwz_a
@@ -590,7 +591,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     // loads in flight.
     bool ok;
     auto chunk_index = [&](const P2GHeader& hh, bool exists, int c) -> uint32_t {
-        ok = exists && lane < hh.nb - c;
+        ok = exists && lane < kP2GChunk && lane < hh.nb - c;
         return a.bin.sidx[ok ? hh.off + c + lane : 0u];
     };
     P2GRaw raw{};
