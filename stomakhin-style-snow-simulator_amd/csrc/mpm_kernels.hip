@@ -96,99 +96,142 @@ __global__ void k_set_count(Counters* c, uint32_t n) {
 // Consecutive particles mostly share a block and a cell (the buffer is sorted
 // by the previous step), so both are aggregated per wave: one lane inserts
 // each distinct block key, one lane bumps each distinct cell counter by the
-// number of lanes in it, and lanes take consecutive ranks.
+// number of lanes in it, and lanes take consecutive ranks.  A wave takes two
+// sets of 64 consecutive particles per pass, whose hash lookups and atomics
+// are in flight together (set 0 elects the lowest lane of each distinct key,
+// set 1 the highest, so a lane seldom serves both), and the positions of the
+// next pass are loaded meanwhile.
+struct BinItem {
+    uint64_t key;
+    uint32_t l, sid;
+    bool want;
+};
+__device__ __forceinline__ BinItem bin_item(const StepArgs& a, uint32_t i, uint32_t n, uint32_t sid, float x0, float x1,
+                                            float x2) {
+    BinItem it{kEmptyKey, 0u, sid, false};
+    if (i >= n) return it;
+    const SceneDev& sc = a.scenes[sid];
+    if (!(sc.flags & a.pass_flag)) return it;
+    if (!finite3(x0, x1, x2)) {
+        flag(a.st, sid, kStatusNaN);
+        return it;
+    }
+    int b[3];
+    float f[3];
+    base_node(sc, x0, x1, x2, b, f);
+    const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
+    const int lim = kCoordBias - 2;
+    if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
+        flag(a.st, sid, kStatusRange);
+        return it;
+    }
+    it.l = (uint32_t)(b[0] & 3) | ((uint32_t)(b[1] & 3) << 2) | ((uint32_t)(b[2] & 3) << 4);
+    it.key = pack_key(sid, bx, by, bz);
+    it.want = true;
+    return it;
+}
+// Leader lane of this lane's 64-bit value among the lanes with want set:
+// the lowest (HIGH = false) or highest lane holding the same value.
+template <bool HIGH>
+__device__ __forceinline__ int bin_leader64(uint64_t v, bool want) {
+    int leader = (int)(threadIdx.x & 63);
+    uint64_t pending = __ballot(want);
+    while (pending) {
+        const int l = HIGH ? 63 - __clzll((long long)pending) : __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lo = __shfl((uint32_t)v, l), hi = __shfl((uint32_t)(v >> 32), l);
+        const bool match = want && v == (((uint64_t)hi << 32) | lo);
+        if (match) leader = l;
+        pending &= ~__ballot(match);
+    }
+    return leader;
+}
+template <bool HIGH>
+__device__ __forceinline__ int bin_leader32(uint32_t v, bool want, uint64_t& group) {
+    int leader = (int)(threadIdx.x & 63);
+    group = 0;
+    uint64_t pending = __ballot(want);
+    while (pending) {
+        const int l = HIGH ? 63 - __clzll((long long)pending) : __ffsll((unsigned long long)pending) - 1;
+        const bool match = want && v == __shfl(v, l);
+        const uint64_t mm = __ballot(match);
+        if (match) {
+            leader = l;
+            group = mm;
+        }
+        pending &= ~mm;
+    }
+    return leader;
+}
+
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // The scene id and position of the next pass are loaded while this pass
-    // waits on its hash lookup and cell-count atomics.
-    uint32_t wbase = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-    uint32_t nsid = 0;
-    float nx0 = 0.f, nx1 = 0.f, nx2 = 0.f;
-    if (wbase + lane < n) {
-        nsid = a.cur.sid[wbase + lane];
-        nx0 = a.cur.x[0][wbase + lane];
-        nx1 = a.cur.x[1][wbase + lane];
-        nx2 = a.cur.x[2][wbase + lane];
-    }
+    const uint32_t stride = gridDim.x * blockDim.x * 2;
+    uint32_t wbase = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) * 2;
+    uint32_t nsid[2] = {0u, 0u};
+    float nx[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    auto load = [&](uint32_t base) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t i = base + 64 * s + lane;
+            if (i < n) {
+                nsid[s] = a.cur.sid[i];
+                nx[s][0] = a.cur.x[0][i];
+                nx[s][1] = a.cur.x[1][i];
+                nx[s][2] = a.cur.x[2][i];
+            }
+        }
+    };
+    load(wbase);
     for (; wbase < n; wbase += stride) {
-        const uint32_t i = wbase + lane;
-        const float x0 = nx0, x1 = nx1, x2 = nx2;
-        uint64_t key = kEmptyKey;
-        uint32_t l = 0, sid = nsid;
-        bool want = false;
-        if (wbase + stride + lane < n) {
-            nsid = a.cur.sid[wbase + stride + lane];
-            nx0 = a.cur.x[0][wbase + stride + lane];
-            nx1 = a.cur.x[1][wbase + stride + lane];
-            nx2 = a.cur.x[2][wbase + stride + lane];
+        BinItem it[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) it[s] = bin_item(a, wbase + 64 * s + lane, n, nsid[s], nx[s][0], nx[s][1], nx[s][2]);
+        load(wbase + stride);
+        // block keys: one insert per distinct key and set
+        const int kl0 = bin_leader64<false>(it[0].key, it[0].want);
+        const int kl1 = bin_leader64<true>(it[1].key, it[1].want);
+        const bool do0 = it[0].want && kl0 == (int)lane, do1 = it[1].want && kl1 == (int)lane;
+        uint32_t s0 = kInvalid, s1 = kInvalid;
+        bool ins;
+        if (do0 && do1) {
+            s0 = hash_insert(a.g, it[0].key, ins);
+            s1 = hash_insert(a.g, it[1].key, ins);
+        } else if (do0 || do1) {
+            const uint32_t sl = hash_insert(a.g, do0 ? it[0].key : it[1].key, ins);
+            (do0 ? s0 : s1) = sl;
         }
-        if (i < n) {
-            const SceneDev& sc = a.scenes[sid];
-            if (sc.flags & a.pass_flag) {
-                if (!finite3(x0, x1, x2)) {
-                    flag(a.st, sid, kStatusNaN);
-                } else {
-                    int b[3];
-                    float f[3];
-                    base_node(sc, x0, x1, x2, b, f);
-                    const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
-                    const int lim = kCoordBias - 2;
-                    if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
-                        flag(a.st, sid, kStatusRange);
-                    } else {
-                        l = (uint32_t)(b[0] & 3) | ((uint32_t)(b[1] & 3) << 2) | ((uint32_t)(b[2] & 3) << 4);
-                        key = pack_key(sid, bx, by, bz);
-                        want = true;
-                    }
-                }
-            }
+        if (do0 && s0 == kInvalid) flag(a.st, it[0].sid, kStatusHashFull);
+        if (do1 && s1 == kInvalid) flag(a.st, it[1].sid, kStatusHashFull);
+        const uint32_t slot0 = it[0].want ? __shfl(s0, kl0) : kInvalid;
+        const uint32_t slot1 = it[1].want ? __shfl(s1, kl1) : kInvalid;
+        // cells: one counter bump per distinct cell and set
+        const bool h0 = slot0 != kInvalid, h1 = slot1 != kInvalid;
+        const uint32_t ck0 = h0 ? (slot0 << 6) | it[0].l : kInvalid;
+        const uint32_t ck1 = h1 ? (slot1 << 6) | it[1].l : kInvalid;
+        uint64_t g0, g1;
+        const int cl0 = bin_leader32<false>(ck0, h0, g0);
+        const int cl1 = bin_leader32<true>(ck1, h1, g1);
+        const bool c0 = h0 && cl0 == (int)lane, c1 = h1 && cl1 == (int)lane;
+        uint32_t b0 = 0u, b1 = 0u;
+        if (c0 && c1) {
+            b0 = atomicAdd(&a.g.cellcnt[ck0], (uint32_t)__popcll(g0));
+            b1 = atomicAdd(&a.g.cellcnt[ck1], (uint32_t)__popcll(g1));
+        } else if (c0 || c1) {
+            const uint32_t b = atomicAdd(&a.g.cellcnt[c0 ? ck0 : ck1], (uint32_t)__popcll(c0 ? g0 : g1));
+            (c0 ? b0 : b1) = b;
         }
-        // Leaders: the lowest lane of each distinct block key (ballots only),
-        // then every leader inserts in the same instruction.
-        int kleader = (int)lane;
-        uint64_t pending = __ballot(want);
-        while (pending) {
-            const int leader = __ffsll((unsigned long long)pending) - 1;
-            const uint32_t klo = __shfl((uint32_t)key, leader), khi = __shfl((uint32_t)(key >> 32), leader);
-            const bool match = want && key == (((uint64_t)khi << 32) | klo);
-            if (match) kleader = leader;
-            pending &= ~__ballot(match);
+        b0 = __shfl(b0, cl0);
+        b1 = __shfl(b1, cl1);
+        const uint32_t i0 = wbase + lane, i1 = wbase + 64 + lane;
+        if (i0 < n) {
+            a.bin.pkey[i0] = ck0;
+            a.bin.prank[i0] = h0 ? b0 + (uint32_t)__popcll(g0 & lt_mask) : 0u;
         }
-        uint32_t s_ = kInvalid;
-        if (want && kleader == (int)lane) {
-            bool ins;
-            s_ = hash_insert(a.g, key, ins);
-            if (s_ == kInvalid) flag(a.st, sid, kStatusHashFull);
-        }
-        const uint32_t slot_b = __shfl(s_, kleader);
-        const uint32_t slot = want ? slot_b : kInvalid;
-        // Same for cells: one counter bump per distinct cell, all at once.
-        const bool have = slot != kInvalid;
-        const uint32_t ck = have ? (slot << 6) | l : kInvalid;
-        int cleader = (int)lane;
-        uint64_t cmask = 0;
-        uint64_t pend2 = __ballot(have);
-        while (pend2) {
-            const int leader = __ffsll((unsigned long long)pend2) - 1;
-            const uint32_t lck = __shfl(ck, leader);
-            const bool match = have && ck == lck;
-            const uint64_t mm = __ballot(match);
-            if (match) {
-                cleader = leader;
-                cmask = mm;
-            }
-            pend2 &= ~mm;
-        }
-        uint32_t basecnt = 0;
-        if (have && cleader == (int)lane) basecnt = atomicAdd(&a.g.cellcnt[ck], (uint32_t)__popcll(cmask));
-        basecnt = __shfl(basecnt, cleader);
-        const uint32_t r = have ? basecnt + (uint32_t)__popcll(cmask & lt_mask) : 0u;
-        if (i < n) {
-            a.bin.pkey[i] = ck;
-            a.bin.prank[i] = r;
+        if (i1 < n) {
+            a.bin.pkey[i1] = ck1;
+            a.bin.prank[i1] = h1 ? b1 + (uint32_t)__popcll(g1 & lt_mask) : 0u;
         }
     }
 }
