@@ -88,7 +88,7 @@ inline PBuf compose_pbuf(const PBuf& moving, const PBuf& constant) {
 struct Counters {
     uint32_t n_particles;  // particles in the current buffer
     uint32_t n_home;       // blocks holding particles
-    uint32_t n_all;        // home + neighbour blocks
+    uint32_t n_extra;      // neighbour blocks k_blocks inserted (all_list = home list, then these)
     uint32_t cursor;       // particles binned this pass
     uint32_t n_fallback;   // particles deferred to the FP64 fix-up this pass
     uint32_t global_status;
@@ -110,7 +110,7 @@ struct GridDev {
     uint32_t pad_;
     float* vu;             // [H*6*64] grid velocity and PIC/FLIP blend
     uint32_t* home_list;   // [H]
-    uint32_t* all_list;    // [H]
+    uint32_t* all_list;    // [H] home blocks (same order as home_list), then the other neighbour blocks
 };
 
 struct BinDev {

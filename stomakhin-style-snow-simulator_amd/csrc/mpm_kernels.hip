@@ -2,9 +2,11 @@
 //
 // One step is the reference's step() (src/grid.cpp:359-450) restructured for
 // the GPU:
-//   k_begin     reset per-pass counters
 //   k_bin       home block of each particle -> device hash, per-cell counts
-//   k_blocks    per home block: cell prefix sums, particle range, neighbour blocks
+//               (and the per-pass counters are reset)
+//   k_list      compact the occupied hash slots into the home-block list
+//   k_blocks    per home block: cell prefix sums, particle range, neighbour
+//               blocks (appended to the list of all blocks)
 //   k_scatter   counting-sort scatter of particle indices by (block, cell)
 //   k_cellsort  order each cell by particle id (makes P2G order canonical)
 //   k_p2g       P2G of mass, momentum and stress force (grid.cpp:119-129,
@@ -15,7 +17,8 @@
 //   k_g2p       stress_update + G2P + particle collisions + advection
 //               (grid.cpp:245-338) fused, also producing the next step's
 //               force matrix -V0 P F_E^T so P2G needs no SVD
-//   k_end       clear the hash slots used this pass, publish the particle count
+//   k_end       FP64 redo of the flagged plasticity updates, clear the hash
+//               slots used this pass, publish the particle count
 #include <hip/hip_runtime.h>
 
 #include "mpm_device.hpp"
@@ -77,14 +80,7 @@ __device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1
 }
 
 // ---------------------------------------------------------------------------
-__global__ void k_begin(StepArgs a) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        a.cnt->n_home = 0;
-        a.cnt->n_all = 0;
-        a.cnt->cursor = 0;
-        a.cnt->n_fallback = 0;
-    }
-}
+
 
 
 
@@ -165,6 +161,12 @@ __device__ __forceinline__ int bin_leader32(uint32_t v, bool want, uint64_t& gro
 
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
+    if (threadIdx.x == 0 && blockIdx.x == 0) {  // per-pass counters (no kernel of this pass has read them yet)
+        a.cnt->n_home = 0;
+        a.cnt->n_extra = 0;
+        a.cnt->cursor = 0;
+        a.cnt->n_fallback = 0;
+    }
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t stride = gridDim.x * blockDim.x * 2;
@@ -236,13 +238,13 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     }
 }
 
-// Compacts the occupied hash slots into a list (after k_bin: the home
-// blocks; after k_blocks: every block).  One atomic per 4096-slot tile.
-__global__ __launch_bounds__(1024) void k_list(StepArgs a, int all) {
+// Compacts the occupied hash slots into the list of home blocks (after
+// k_bin every occupied slot is a home block).  One atomic per 4096-slot tile.
+__global__ __launch_bounds__(1024) void k_list(StepArgs a) {
     __shared__ uint32_t wsum[16];
     __shared__ uint32_t base_s;
-    uint32_t* list = all ? a.g.all_list : a.g.home_list;
-    uint32_t* counter = all ? &a.cnt->n_all : &a.cnt->n_home;
+    uint32_t* list = a.g.home_list;
+    uint32_t* counter = &a.cnt->n_home;
     const uint32_t H = a.g.mask + 1;
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (uint32_t tile = blockIdx.x * 4096u; tile < H; tile += gridDim.x * 4096u) {
@@ -282,7 +284,9 @@ __global__ __launch_bounds__(1024) void k_list(StepArgs a, int all) {
 
 // Per home block (one wave): exclusive scan of the 64 cell counts, particle
 // range, and the 8 blocks its nodes live in (the block itself + {0,1}^3).
-// A workgroup of 16 waves takes one particle range for its 16 blocks.
+// A workgroup of 16 waves takes one particle range for its 16 blocks.  The
+// list of all blocks is the home list followed by the neighbour blocks this
+// kernel inserts (each appended by the lane whose insert created it).
 __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
     __shared__ uint32_t tot[16];
     __shared__ uint32_t base_s;
@@ -314,7 +318,11 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                     if (ns == kInvalid) {
                         flag(a.st, sid, kStatusHashFull);
                         ns = slot;  // keep indices in range; the pass is flagged as failed
+                    } else if (ins) {
+                        a.g.all_list[nh + atomicAdd(&a.cnt->n_extra, 1u)] = ns;
                     }
+                } else {
+                    a.g.all_list[b] = slot;
                 }
                 a.g.nb[(size_t)slot * 8 + lane] = ns;
                 // The block H + t learns that home block H (home-list index b)
@@ -786,7 +794,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
 __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int lx = (int)(lane & 3), ly = (int)((lane >> 2) & 3), lz = (int)(lane >> 4);
-    const uint32_t na = a.cnt->n_all;
+    const uint32_t na = a.cnt->n_home + a.cnt->n_extra;
     for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
         const uint32_t slot = a.g.all_list[b];
         uint32_t sid;
@@ -1274,7 +1282,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                                            sc.hardening, V0, a.kappa, po, true);
             }
             if (fb) {
-                // Redone in FP64 by k_fp64_fixup; keep finite placeholders meanwhile.
+                // Redone in FP64 at the start of k_end; keep finite placeholders meanwhile.
                 const uint32_t q = atomicAdd(&a.cnt->n_fallback, 1u);
                 a.fb.pos[q] = j;
 #pragma unroll
@@ -1341,7 +1349,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 // (inverted, singular or cond(trial F_E) > kappa).  Reads the pre-step
 // state through the sorted index (G2P left the old F_E, J_P and F_P there
 // for these particles) and writes the post-step state where G2P wrote.
-__global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
+__device__ __forceinline__ void fp64_fixup(const StepArgs& a) {
     const uint32_t nf = a.cnt->n_fallback;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < nf; q += gridDim.x * blockDim.x) {
         const uint32_t j = a.fb.pos[q];
@@ -1373,10 +1381,12 @@ __global__ __launch_bounds__(256) void k_fp64_fixup(StepArgs a) {
     }
 }
 
-// Clear the hash slots used this pass; after a permuting pass publish the
-// new (compacted) particle count.
+// End of a pass: the FP64 fix-ups, then clear the hash slots used this pass;
+// after a permuting pass publish the new (compacted) particle count.  (The
+// fix-ups touch only particle buffers, the clearing only the grid.)
 __global__ __launch_bounds__(256) void k_end(StepArgs a, int publish) {
-    const uint32_t na = a.cnt->n_all;
+    fp64_fixup(a);
+    const uint32_t na = a.cnt->n_home + a.cnt->n_extra;
     for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < na; b += gridDim.x * blockDim.x) {
         const uint32_t slot = a.g.all_list[b];
         a.g.keys[slot] = kEmptyKey;
@@ -1530,11 +1540,9 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
 
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], s);
-    k_begin<<<1, 64, 0, s>>>(a);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
-    k_list<<<c.grid_l, 1024, 0, s>>>(a, 0);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a);
     k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
-    k_list<<<c.grid_l, 1024, 0, s>>>(a, 1);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -1553,18 +1561,15 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
         else
             k_g2p<false, false><<<c.grid_w, 256, 0, s>>>(a);
     }
-    k_fp64_fixup<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[4], s);
     k_end<<<c.grid_b, 256, 0, s>>>(a, a.permute);
     if (ev) (void)hipEventRecord(ev[5], s);
 }
 
 void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
-    k_begin<<<1, 64, 0, s>>>(a);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
-    k_list<<<c.grid_l, 1024, 0, s>>>(a, 0);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a);
     k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
-    k_list<<<c.grid_l, 1024, 0, s>>>(a, 1);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     k_p2g<<<c.grid_w * 4, 64, 0, s>>>(a);
