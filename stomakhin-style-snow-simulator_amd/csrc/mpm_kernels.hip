@@ -795,11 +795,27 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int lx = (int)(lane & 3), ly = (int)((lane >> 2) & 3), lz = (int)(lane >> 4);
     const uint32_t na = a.cnt->n_home + a.cnt->n_extra;
-    for (uint32_t b = blockIdx.x * 4 + wid; b < na; b += gridDim.x * 4) {
-        const uint32_t slot = a.g.all_list[b];
+    const uint32_t stride = gridDim.x * 4;
+    uint32_t b = blockIdx.x * 4 + wid;
+    // The next block's slot, key and home-block indices are loaded while
+    // this block sums its partials.
+    uint32_t nslot = 0, nrev = kInvalid;
+    uint64_t nkey = 0;
+    if (b < na) {
+        nslot = a.g.all_list[b];
+        nkey = a.g.keys[nslot];
+        nrev = lane < 8 ? a.g.rev[(size_t)nslot * 8 + lane] : kInvalid;
+    }
+    for (; b < na; b += stride) {
+        const uint32_t slot = nslot, revl = nrev;
         uint32_t sid;
         int bx, by, bz;
-        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        unpack_key(nkey, sid, bx, by, bz);
+        if (b + stride < na) {
+            nslot = a.g.all_list[b + stride];
+            nkey = a.g.keys[nslot];
+            nrev = lane < 8 ? a.g.rev[(size_t)nslot * 8 + lane] : kInvalid;
+        }
         const SceneDev& sc = a.scenes[sid];
         float q[kGridCh];
 #pragma unroll
@@ -807,7 +823,7 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int nx = piece_n(t, 0), ny = piece_n(t, 1), nz = piece_n(t, 2), sz = nx * ny * nz;
-            const uint32_t hb = a.g.rev[(size_t)slot * 8 + t];
+            const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)revl, t);
             if (hb != kInvalid && lx < nx && ly < ny && lz < nz) {
                 const float* src =
                     a.g.part + (size_t)hb * kGridCh * kRegion + kGridCh * piece_base(t) + (lz * ny + ly) * nx + lx;
