@@ -391,11 +391,21 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
     __shared__ uint32_t ki[4][kSortCap];
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t b = blockIdx.x * 4 + wid; b < nh; b += gridDim.x * 4) {
-        const uint32_t slot = a.g.home_list[b];
-        const uint32_t off = a.g.blkoff[slot], nb = a.g.blkcnt[slot];
-        const uint32_t cnt = a.g.cellcnt[(size_t)slot * 64 + lane];
-        const uint32_t c0 = a.g.celloff[(size_t)slot * 64 + lane];
+    const uint32_t stride = gridDim.x * 4;
+    uint32_t b = blockIdx.x * 4 + wid;
+    // The next block's records are loaded while this block is sorted.
+    uint32_t noff = 0, nnb = 0, ncnt = 0, nc0 = 0;
+    auto records = [&](uint32_t bb) {
+        const uint32_t slot = a.g.home_list[bb];
+        noff = a.g.blkoff[slot];
+        nnb = a.g.blkcnt[slot];
+        ncnt = a.g.cellcnt[(size_t)slot * 64 + lane];
+        nc0 = a.g.celloff[(size_t)slot * 64 + lane];
+    };
+    if (b < nh) records(b);
+    for (; b < nh; b += stride) {
+        const uint32_t off = noff, nb = nnb, cnt = ncnt, c0 = nc0;
+        if (b + stride < nh) records(b + stride);
         if (nb <= (uint32_t)kSortCap) {
             uint32_t* P = kp[wid];
             uint32_t* I = ki[wid];
