@@ -288,15 +288,18 @@ __global__ __launch_bounds__(1024) void k_list(StepArgs a) {
 // list of all blocks is the home list followed by the neighbour blocks this
 // kernel inserts (each appended by the lane whose insert created it).
 __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
-    __shared__ uint32_t tot[16];
-    __shared__ uint32_t base_s;
+    __shared__ uint32_t tot[16], ext[16];
+    __shared__ uint32_t base_s, ext_s;
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t nh = a.cnt->n_home;
     for (uint32_t b0 = blockIdx.x * 16; b0 < nh; b0 += gridDim.x * 16) {
         const uint32_t b = b0 + wid;
         const bool valid = b < nh;
         const uint32_t slot = valid ? a.g.home_list[b] : 0u;
         uint32_t total = 0;
+        uint32_t ns = slot;
+        bool ins = false;
         if (valid) {
             const uint32_t c = a.g.cellcnt[(size_t)slot * 64 + lane];
             uint32_t s = c;
@@ -308,18 +311,15 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
             a.g.celloff[(size_t)slot * 64 + lane] = s - c;
             total = __shfl(s, 63, 64);
             if (lane < 8) {
-                uint32_t ns = slot;
                 if (lane != 0) {
                     uint32_t sid;
                     int bx, by, bz;
                     unpack_key(a.g.keys[slot], sid, bx, by, bz);
-                    bool ins;
                     ns = hash_insert(a.g, pack_key(sid, bx + (int)(lane & 1), by + (int)((lane >> 1) & 1), bz + (int)(lane >> 2)), ins);
                     if (ns == kInvalid) {
                         flag(a.st, sid, kStatusHashFull);
                         ns = slot;  // keep indices in range; the pass is flagged as failed
-                    } else if (ins) {
-                        a.g.all_list[nh + atomicAdd(&a.cnt->n_extra, 1u)] = ns;
+                        ins = false;
                     }
                 } else {
                     a.g.all_list[b] = slot;
@@ -330,22 +330,33 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                 if (b < a.g.max_home) a.g.rev[(size_t)ns * 8 + lane] = b;
             }
         }
-        if (lane == 0) tot[wid] = total;
+        // Neighbour blocks created here are appended to the all-blocks list
+        // with one counter update per workgroup (a counter bumped by every
+        // wave serialises at one L2 channel).
+        const uint64_t imask = __ballot(ins);
+        if (lane == 0) {
+            tot[wid] = total;
+            ext[wid] = (uint32_t)__popcll(imask);
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
-            uint32_t run = 0;
+            uint32_t run = 0, erun = 0;
             for (int k = 0; k < 16; ++k) {
-                const uint32_t v = tot[k];
+                const uint32_t v = tot[k], e = ext[k];
                 tot[k] = run;
+                ext[k] = erun;
                 run += v;
+                erun += e;
             }
             base_s = run ? atomicAdd(&a.cnt->cursor, run) : 0u;
+            ext_s = erun ? atomicAdd(&a.cnt->n_extra, erun) : 0u;
         }
         __syncthreads();
         if (valid && lane == 0) {
             a.g.blkoff[slot] = base_s + tot[wid];
             a.g.blkcnt[slot] = total;
         }
+        if (ins) a.g.all_list[nh + ext_s + ext[wid] + (uint32_t)__popcll(imask & lt_mask)] = ns;
         __syncthreads();
     }
 }
@@ -468,8 +479,8 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // ch = (l>>2)&3, and holds the x nodes r = 0..3 of node (., k, l&3),
 // channel (l>>2)&3.
 //
-// A wave stages its block's particles (sorted: cell, then particle id) 64 at
-// a time into its own LDS records (per-axis B-spline weights and derivatives
+// A wave stages its block's particles (sorted: cell, then particle id) 32 at
+// a time (kP2GChunk) into its own LDS records (per-axis B-spline weights and derivatives
 // of grid.cpp:24-58 in grid units, m, m v, S/dx): a lane fetches its x/z
 // terms with one b128; the six y/channel terms are the same for the 4 lanes
 // of a quad, which fetch them with one b64 each and share them by DPP quad
