@@ -13,7 +13,10 @@ The line also carries the batched form (256 uneven scenes, graph-replayed,
 autotuned) and the served form (Poisson arrivals at 10-200 requests/s,
 continuous batching) with p50/p99 latency, beside the plain per-scene GPU step.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving]
+The seeding entry (mesh -> lattice particles, the reference's Blender add-on
+restated on the GPU) is reported beside them with its own CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving] [--no-seeding]
 """
 import argparse
 import json
@@ -151,6 +154,63 @@ def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(1
     return out
 
 
+def _torus_mesh(R, r, nu, nv):
+    """Closed torus around the z axis: nu x nv quads as 2 nu nv triangles."""
+    import numpy as np
+    u = 2 * np.pi * np.arange(nu) / nu
+    w = 2 * np.pi * np.arange(nv) / nv
+    U, W = np.meshgrid(u, w, indexing="ij")
+    v = np.stack([(R + r * np.cos(W)) * np.cos(U), (R + r * np.cos(W)) * np.sin(U), r * np.sin(W)], -1).reshape(-1, 3)
+    i, j = np.meshgrid(np.arange(nu), np.arange(nv), indexing="ij")
+    a, b = i * nv + j, ((i + 1) % nu) * nv + j
+    c, d = ((i + 1) % nu) * nv + (j + 1) % nv, i * nv + (j + 1) % nv
+    f = np.concatenate([np.stack([a, b, c], -1).reshape(-1, 3), np.stack([a, c, d], -1).reshape(-1, 3)])
+    return v, f.astype(np.int32)
+
+
+def seeding(no_cpu):
+    """Mesh seeding (the Blender add-on's seed_mesh, on the GPU): a torus of
+    9,216 triangles on a 1.35M-point lattice; the CPU baseline is the oracle
+    on the same mesh at a coarser spacing (the work per lattice point does
+    not depend on the spacing)."""
+    import numpy as np
+
+    import snowmpm
+    v, f = _torus_mesh(0.3, 0.12, 96, 48)
+    spacing = 0.005
+    snowmpm.seed_mesh(v, f, spacing)  # warm-up (module load, allocations)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        pts = snowmpm.seed_mesh(v, f, spacing)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    lo, hi = v.astype(np.float32).min(0), v.astype(np.float32).max(0)
+    s32, npts = float(np.float32(spacing)), 1
+    for e in range(3):  # lattice size (seed_export.py:109-127)
+        k, x = 0, float(lo[e]) + s32 * 0.5
+        while x < float(hi[e]):
+            k, x = k + 1, x + s32
+        npts *= k
+    out = {"mesh": "torus R=0.3 r=0.12, 9216 triangles", "spacing": spacing, "lattice_points": npts,
+           "particles": int(pts.shape[0]), "call_ms": best * 1e3, "lattice_points_per_s": npts / best,
+           "note": "whole call: host lattice, uploads, kernel, download, compaction"}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc  # timed CPU baseline only
+        cs = 0.03
+        t0 = time.perf_counter()
+        ref = orc.seed_mesh(v, f, cs)
+        dt = time.perf_counter() - t0
+        nc = 1
+        for e in range(3):
+            nc *= orc.seed_axis(float(lo[e]), float(hi[e]), cs).shape[0]
+        out["cpu_baseline"] = {"value": nc / dt, "unit": "lattice points/s", "cores": 1, "kind": "port",
+                               "sample": f"same mesh at spacing {cs}: {nc} lattice points, {ref.shape[0]} "
+                                         f"particles, scalar oracle, {dt:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,6 +219,7 @@ def main():
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched / served measurements")
+    ap.add_argument("--no-seeding", action="store_true", help="skip the mesh-seeding measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -232,6 +293,8 @@ def main():
     }
     if not args.no_serving:
         out["batched"] = batched_and_served(args.seed)
+    if not args.no_seeding:
+        out["seeding"] = seeding(args.no_cpu)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:

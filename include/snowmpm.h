@@ -208,6 +208,20 @@ void mpm_free(void* p);
 int mpm_scene_generate(int kind, uint64_t seed, int index, double scale, size_t* n, double** x, double** v,
                        double** m, double** E, mpm_params* prm, mpm_collider* colliders, int* ncolliders);
 
+/* ---- seeding: lattice particles inside a closed triangle mesh ----
+ * Replaces seed_mesh / point_inside_mesh / ray_inside / world_bounds of the
+ * reference's Blender add-on (blender/seed_export.py:43-131).  verts: nverts*3
+ * coordinates in the mesh's frame (rounded to float32, as Blender stores
+ * them); tris: ntris*3 vertex indices (polygons already triangulated);
+ * spacing > 0 (rounded to float32, like the add-on's spacing property).
+ * Lattice points min + spacing/2 + k spacing per axis inside the mesh's
+ * bounds are kept when at least two of three fixed rays cross the surface an
+ * odd number of times; the inside test runs on GPU `device`.  On success *n is
+ * the count and *x (n*3, malloc'ed, release with mpm_free; NULL when n = 0)
+ * the points as float32 values, x outer and z inner, in the mesh's frame. */
+int mpm_seed_mesh(int device, const double* verts, size_t nverts, const int32_t* tris, size_t ntris, double spacing,
+                  size_t* n, double** x);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,6 +82,10 @@ def _load():
                             C.POINTER(Timings), i, G]),
         "orc_run": (None, [P, C.c_size_t, C.POINTER(Collider), i, C.POINTER(Params), i,
                            C.POINTER(Timings), i]),
+        "orc_seed_axis": (C.c_size_t, [C.c_float, C.c_float, d, C.POINTER(C.c_float), C.c_size_t]),
+        "orc_seed_direction": (None, [i, C.POINTER(C.c_float)]),
+        "orc_seed_ray_hits": (i, [pd, C.c_size_t, C.POINTER(C.c_int), C.c_size_t, pd, i]),
+        "orc_seed_mesh": (C.c_long, [pd, C.c_size_t, C.POINTER(C.c_int), C.c_size_t, d, pd, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -261,3 +265,42 @@ class State:
         arr, nc = colliders_array(colliders)
         lib.orc_run(self.arr, self.n, arr, nc, C.byref(prm), int(nsteps),
                     C.byref(timings) if timings else None, int(nthreads))
+
+
+# ---- mesh seeding (seed_oracle.cpp, blender/seed_export.py:43-131) ----
+
+def _mesh(verts, tris):
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).reshape(-1, 3))
+    return v, t, v.ctypes.data_as(C.POINTER(C.c_double)), t.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def seed_axis(lo, hi, spacing):
+    """Lattice coordinates of one axis (float32 values)."""
+    n = lib.orc_seed_axis(float(np.float32(lo)), float(np.float32(hi)), float(spacing), None, 0)
+    out = np.zeros(n, dtype=np.float32)
+    lib.orc_seed_axis(float(np.float32(lo)), float(np.float32(hi)), float(spacing),
+                      out.ctypes.data_as(C.POINTER(C.c_float)), n)
+    return out
+
+
+def seed_direction(k):
+    d = (C.c_float * 3)()
+    lib.orc_seed_direction(int(k), d)
+    return np.array(list(d), dtype=np.float32)
+
+
+def seed_ray_hits(verts, tris, p, k):
+    v, t, pv, pt = _mesh(verts, tris)
+    return lib.orc_seed_ray_hits(pv, v.shape[0], pt, t.shape[0], vec(*map(float, p)), int(k))
+
+
+def seed_mesh(verts, tris, spacing):
+    """Seeded points (n x 3, float64 holding float32 values), x outer, z inner."""
+    v, t, pv, pt = _mesh(verts, tris)
+    n = lib.orc_seed_mesh(pv, v.shape[0], pt, t.shape[0], float(spacing), None, 0)
+    if n < 0:
+        raise ValueError("bad mesh or spacing")
+    out = np.zeros((n, 3))
+    lib.orc_seed_mesh(pv, v.shape[0], pt, t.shape[0], float(spacing), out.ctypes.data_as(C.POINTER(C.c_double)), n)
+    return out

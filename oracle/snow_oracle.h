@@ -129,6 +129,19 @@ void orc_step(orc_particle* ps, size_t n, const orc_collider* c, int nc,
 void orc_run(orc_particle* ps, size_t n, const orc_collider* c, int nc,
              const orc_params* prm, int nsteps, orc_timings* t, int nthreads);
 
+/* ---- mesh seeding (blender/seed_export.py:43-131), seed_oracle.cpp ----
+ * verts: nverts x 3 (stored as float32, like Blender's mesh), tris: ntris x 3
+ * vertex indices.  orc_seed_axis: lattice coordinates of one axis (returns
+ * the count; writes at most cap).  orc_seed_direction: ray direction k.
+ * orc_seed_ray_hits: hits counted along ray k from point p (-1: bad input).
+ * orc_seed_mesh: the seeded points (x outer, z inner), count returned, at
+ * most cap written to out (n x 3); -1 on bad input. */
+size_t orc_seed_axis(float lo, float hi, double spacing, float* out, size_t cap);
+void orc_seed_direction(int k, float d[3]);
+int orc_seed_ray_hits(const double* verts, size_t nverts, const int* tris, size_t ntris, const double p[3], int k);
+long orc_seed_mesh(const double* verts, size_t nverts, const int* tris, size_t ntris, double spacing, double* out,
+                   size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

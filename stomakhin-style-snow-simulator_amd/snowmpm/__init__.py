@@ -111,6 +111,8 @@ _SIGS = {
     "mpm_scene_generate": (C.c_int, [C.c_int, C.c_uint64, C.c_int, C.c_double, C.POINTER(C.c_size_t),
                                      _ppd, _ppd, _ppd, _ppd, C.POINTER(Params), C.POINTER(Collider),
                                      C.POINTER(C.c_int)]),
+    "mpm_seed_mesh": (C.c_int, [C.c_int, _pd, C.c_size_t, C.POINTER(C.c_int32), C.c_size_t, C.c_double,
+                                C.POINTER(C.c_size_t), _ppd]),
 }
 for _name, (_res, _args) in _SIGS.items():
     _f = getattr(lib, _name)
@@ -213,6 +215,25 @@ def export_frame(directory, frame_no, x):
     """export_particles (export.cpp:10-22) into directory/frame_%06d.csv."""
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
     _check(lib.mpm_export_frame(os.fsencode(directory), int(frame_no), x.shape[0], _dptr(x)))
+
+
+def seed_mesh(verts, tris, spacing, device=0):
+    """Lattice particles inside a closed triangle mesh, on the GPU.
+
+    seed_mesh / point_inside_mesh / ray_inside of the reference's Blender
+    add-on (blender/seed_export.py:43-131): verts (n x 3) and tris (k x 3
+    vertex indices) in the mesh's frame; returns the points (m x 3, float32
+    values held as float64), x outer and z inner, in that frame.
+    """
+    v = np.ascontiguousarray(np.asarray(verts, dtype=np.float64).reshape(-1, 3))
+    t = np.ascontiguousarray(np.asarray(tris, dtype=np.int32).reshape(-1, 3))
+    n = C.c_size_t()
+    x = _pd()
+    _check(lib.mpm_seed_mesh(int(device), _dptr(v), v.shape[0], t.ctypes.data_as(C.POINTER(C.c_int32)),
+                             t.shape[0], float(spacing), C.byref(n), C.byref(x)))
+    if n.value == 0:
+        return np.zeros((0, 3))
+    return _take(x, n.value, 3)
 
 
 class Engine:
