@@ -59,13 +59,18 @@ typedef struct {
 
 /* POD collider (replaces CollisionBody, collision.hpp:6-12).  type 0 is the
  * reference's plane (make_plane, collision.cpp:5-27); 1 a solid axis-aligned
- * box; 2 a solid sphere.  velocity: the body's constant velocity. */
+ * box; 2 a solid sphere.  velocity: the body's constant velocity.
+ * flags: MPM_COLLIDER_PROJECT moves a particle that ends a step inside the
+ * body (phi < 0 after advection) onto its surface, x -= phi n.  It has no
+ * counterpart in the reference, whose collisions change velocities only
+ * (grid.cpp:327-338); the factories leave flags 0, the reference behaviour. */
 #define MPM_COLLIDER_PLANE 0
 #define MPM_COLLIDER_BOX 1
 #define MPM_COLLIDER_SPHERE 2
+#define MPM_COLLIDER_PROJECT 0x1
 typedef struct {
     int type;
-    int pad_;
+    int flags;
     double point[3];
     double normal[3];
     double half_extent[3];

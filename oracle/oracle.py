@@ -24,7 +24,7 @@ class Particle(C.Structure):
 
 
 class Collider(C.Structure):
-    _fields_ = [("type", C.c_int), ("pad_", C.c_int), ("point", C.c_double * 3),
+    _fields_ = [("type", C.c_int), ("flags", C.c_int), ("point", C.c_double * 3),
                 ("normal", C.c_double * 3), ("half_extent", C.c_double * 3),
                 ("radius", C.c_double), ("velocity", C.c_double * 3), ("friction", C.c_double)]
 
@@ -77,6 +77,7 @@ def _load():
         "orc_g2p": (None, [G, P, d, d]),
         "orc_particle_collisions": (None, [P, C.POINTER(Collider), i]),
         "orc_position_update": (None, [P, d]),
+        "orc_position_projection": (None, [P, C.POINTER(Collider), i]),
         "orc_setup": (None, [P, C.c_size_t, d]),
         "orc_step": (None, [P, C.c_size_t, C.POINTER(Collider), i, C.POINTER(Params),
                             C.POINTER(Timings), i, G]),
@@ -154,6 +155,12 @@ def make_box(centre, half_extent, friction):
 
 def make_sphere(centre, radius, friction):
     return lib.orc_make_sphere(vec(*centre), float(radius), float(friction))
+
+
+def position_projection(p, cs):
+    """Optional projection of particle p out of the flagged colliders cs."""
+    arr, n = colliders_array(cs)
+    lib.orc_position_projection(C.byref(p), arr, n)
 
 
 def colliders_array(cs):

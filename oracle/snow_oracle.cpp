@@ -502,6 +502,18 @@ void particle_collisions(Particle& p, const orc_collider* cs, int nc) {
 // position_update, grid.cpp:335-338.
 void position_update(Particle& p, double dt) { p.x = add(p.x, scl(dt, p.v)); }
 
+// Optional projection (not in the reference, which only advects): for the
+// colliders with flags bit 0, in list order, a particle with phi(x) < 0 moves
+// to x - phi n.  Velocities are left alone.
+void position_projection(Particle& p, const orc_collider* cs, int nc) {
+    for (int b = 0; b < nc; ++b) {
+        if (!(cs[b].flags & 1)) continue;
+        V3 n;
+        const double phi = coll_phi(cs[b], p.x, &n);
+        if (phi < 0) p.x = sub(p.x, scl(phi, n));
+    }
+}
+
 struct GridBox {
     Grid g;
 };
@@ -577,7 +589,10 @@ void step_impl(std::vector<Particle>& snow, std::vector<Stencil>& stencils, cons
     t->particle_collisions += ms_since(t0, t1);
 
     t0 = clk::now();
-    for (long i = 0; i < n; ++i) position_update(snow[i], dt);
+    for (long i = 0; i < n; ++i) {
+        position_update(snow[i], dt);
+        position_projection(snow[i], cs, nc);
+    }
     t1 = clk::now();
     t->position_update += ms_since(t0, t1);
 
@@ -745,6 +760,11 @@ void orc_particle_collisions(orc_particle* p, const orc_collider* c, int nc) {
 void orc_position_update(orc_particle* p, double dt) {
     Particle q = from_c(*p);
     position_update(q, dt);
+    to_c(q, *p);
+}
+void orc_position_projection(orc_particle* p, const orc_collider* c, int nc) {
+    Particle q = from_c(*p);
+    position_projection(q, c, nc);
     to_c(q, *p);
 }
 

@@ -41,10 +41,12 @@ typedef struct {
 
 /* Collider descriptor.  type 0 = plane (collision.cpp:5-27), 1 = solid
  * axis-aligned box, 2 = solid sphere.  velocity is the collider's constant
- * velocity (the reference plane returns zero, collision.cpp:19-22). */
+ * velocity (the reference plane returns zero, collision.cpp:19-22).
+ * flags bit 0: after position_update, move particles inside the body onto
+ * its surface (an extension; 0 is the reference behaviour). */
 typedef struct {
     int type;
-    int pad_;
+    int flags;
     double point[3];       /* plane: point on plane; box/sphere: centre */
     double normal[3];      /* plane: normal (normalised by orc_make_plane) */
     double half_extent[3]; /* box */
@@ -118,6 +120,8 @@ void orc_stress_update(const orc_grid* g, orc_particle* p, double dx, double dt,
 void orc_g2p(const orc_grid* g, orc_particle* p, double dx, double alpha);
 void orc_particle_collisions(orc_particle* p, const orc_collider* c, int nc);
 void orc_position_update(orc_particle* p, double dt);
+/* Optional projection after position_update (colliders with flags bit 0). */
+void orc_position_projection(orc_particle* p, const orc_collider* c, int nc);
 
 /* ---- whole-path entry points (grid.cpp:340-450) ---- */
 void orc_setup(orc_particle* ps, size_t n, double dx);

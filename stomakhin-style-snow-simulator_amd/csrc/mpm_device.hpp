@@ -32,8 +32,11 @@ constexpr uint32_t kStatusOverflow = 0x2;   // block list capacity exceeded
 constexpr uint32_t kStatusHashFull = 0x4;
 constexpr uint32_t kStatusRange = 0x8;
 
+constexpr int kProjectFlag = 0x1;  // ColliderDev::flags, = MPM_COLLIDER_PROJECT
+
 struct ColliderDev {
-    int type;  // 0 plane, 1 box, 2 sphere
+    int type;   // 0 plane, 1 box, 2 sphere
+    int flags;  // kProjectFlag: project particles that end a step inside
     float point[3];
     float normal[3];
     float half_extent[3];
@@ -51,7 +54,7 @@ struct SceneDev {
     int use_E;                               // per-particle Young's modulus present
     int ncoll;
     int flags;                               // kSceneActive | kSceneSetup
-    int pad_;
+    int nproject;                            // colliders with the projection flag
     ColliderDev coll[kMaxColliders];
 };
 constexpr int kSceneActive = 1;

@@ -174,6 +174,7 @@ StepArgs Engine::args(uint32_t pass_flag, int permute) const {
     a.copy_E = any_E_ ? 1 : 0;
     a.kappa = (float)cfg_.kappa;
     a.permute = permute;
+    a.project = any_project_ ? 1 : 0;
     return a;
 }
 
@@ -293,9 +294,12 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     sd.alpha = (float)prm.alpha;
     sd.use_E = E ? 1 : 0;
     sd.ncoll = ncoll;
+    sd.nproject = 0;
     for (int k = 0; k < ncoll; ++k) {
         ColliderDev& c = sd.coll[k];
         c.type = coll[k].type;
+        c.flags = coll[k].flags & kProjectFlag;
+        sd.nproject += c.flags ? 1 : 0;
         for (int e = 0; e < 3; ++e) {
             c.point[e] = (float)coll[k].point[e];
             c.normal[e] = (float)coll[k].normal[e];
@@ -307,6 +311,7 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     }
     sd.flags = V0 ? kSceneActive : kSceneSetup;
     if (E) any_E_ = true;
+    if (sd.nproject) any_project_ = true;
     if ((rc = upload_scene_table(sid))) return rc;
     HIP_TRY(hipMemsetAsync(st_.scene_status + sid, 0, 4, stream_));
     HIP_TRY(hipMemsetAsync(st_.fp64_count + sid, 0, 8, stream_));
@@ -350,6 +355,7 @@ int Engine::clear() {
     }
     scenes_.clear();
     any_E_ = false;
+    any_project_ = false;
     need_permute_ = true;
     launch_set_count(cnt_, 0, stream_);
     HIP_TRY(hipGetLastError());

@@ -98,6 +98,7 @@ class Engine {
     double* d_sums_ = nullptr;
     void* host_stage_ = nullptr;
     bool any_E_ = false;
+    bool any_project_ = false;  // some scene was added with projecting colliders
     std::vector<SceneDev> h_scenes_;
     std::vector<SceneHost> scenes_;
     std::vector<void*> allocs_;

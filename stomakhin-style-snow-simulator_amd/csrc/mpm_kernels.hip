@@ -1179,7 +1179,7 @@ __device__ __forceinline__ void g2p_stage_store(float4* va, float2* ub, int tg, 
 // its sorted position.
 // Four waves per SIMD: the register budget (<= 128 VGPRs) is met without
 // spills and the extra resident waves hide the particle and grid loads.
-template <bool TRACK, bool PERMUTE>
+template <bool TRACK, bool PERMUTE, bool PROJECT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
     __shared__ float4 va[512];  // [Z][Y][X]: v0 v1 v2 u0
     __shared__ float2 ub[512];  // [Z][Y][X]: u1 u2
@@ -1358,7 +1358,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             v1 = alpha * v1 + pv1;
             v2 = alpha * v2 + pv2;
             collide<float>(sc, x0, x1, x2, v0, v1, v2);           // grid.cpp:327-333, pre-advection x
-            const float nx0 = x0 + dt * v0, nx1 = x1 + dt * v1, nx2 = x2 + dt * v2;  // grid.cpp:335-338
+            float nx0 = x0 + dt * v0, nx1 = x1 + dt * v1, nx2 = x2 + dt * v2;  // grid.cpp:335-338
+            if (PROJECT && sc.nproject) project<float>(sc, nx0, nx1, nx2);
             if (!finite3(nx0, nx1, nx2) || !finite3(v0, v1, v2) || !isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
 
             o.x[0][w] = nx0;
@@ -1575,6 +1576,16 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
 // ---------------------------------------------------------------------------
 // Host launch wrappers.
 
+// G2P with or without the position projection (a branch per particle only
+// when some scene asks for it).
+template <bool TRACK, bool PERMUTE>
+void launch_g2p(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    if (a.project)
+        k_g2p<TRACK, PERMUTE, true><<<c.grid_w, 256, 0, s>>>(a);
+    else
+        k_g2p<TRACK, PERMUTE, false><<<c.grid_w, 256, 0, s>>>(a);
+}
+
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], s);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
@@ -1589,14 +1600,14 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     if (ev) (void)hipEventRecord(ev[3], s);
     if (a.cur.Fp[0]) {
         if (a.permute)
-            k_g2p<true, true><<<c.grid_w, 256, 0, s>>>(a);
+            launch_g2p<true, true>(a, c, s);
         else
-            k_g2p<true, false><<<c.grid_w, 256, 0, s>>>(a);
+            launch_g2p<true, false>(a, c, s);
     } else {
         if (a.permute)
-            k_g2p<false, true><<<c.grid_w, 256, 0, s>>>(a);
+            launch_g2p<false, true>(a, c, s);
         else
-            k_g2p<false, false><<<c.grid_w, 256, 0, s>>>(a);
+            launch_g2p<false, false>(a, c, s);
     }
     if (ev) (void)hipEventRecord(ev[4], s);
     k_end<<<c.grid_b, 256, 0, s>>>(a, a.permute);

@@ -23,6 +23,7 @@ struct StepArgs {
     float kappa;          // FP64 fallback bound on cond(trial F_E)
     int permute;          // 1: G2P writes every particle to nxt in sorted order (and
                           //    compacts); 0: only x v F_E J_P [F_P] S, at index i
+    int project;          // some scene has colliders with position projection
 };
 
 struct LaunchCfg {

@@ -124,6 +124,25 @@ __device__ __forceinline__ void collide(const SceneDev& sc, T x0, T x1, T x2, T&
     }
 }
 
+// Optional position projection after advection (colliders flagged
+// kProjectFlag, in list order): a particle inside the body (phi < 0) moves to
+// x - phi n, its closest surface point.  Not in the reference, which leaves
+// positions to advection (grid.cpp:335-338); velocities are not touched.
+template <typename T>
+__device__ __forceinline__ void project(const SceneDev& sc, T& x0, T& x1, T& x2) {
+    for (int k = 0; k < sc.ncoll; ++k) {
+        const ColliderDev& c = sc.coll[k];
+        if (!(c.flags & kProjectFlag)) continue;
+        T n[3];
+        const T phi = collider_phi<T>(c, x0, x1, x2, n);
+        if (phi < 0) {
+            x0 -= phi * n[0];
+            x1 -= phi * n[1];
+            x2 -= phi * n[2];
+        }
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ T det3(const T A[9]) {
     return A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +

@@ -25,6 +25,7 @@ OK = 0
 ERR_ARG, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_IO = -1, -2, -3, -4, -5, -6, -7
 STATUS_NAN, STATUS_OVERFLOW, STATUS_HASH_FULL, STATUS_RANGE = 0x1, 0x2, 0x4, 0x8
 PLANE, BOX, SPHERE = 0, 1, 2
+PROJECT = 0x1  # Collider.flags: project particles that end a step inside the body onto its surface
 
 
 class MpmError(RuntimeError):
@@ -47,7 +48,7 @@ class Params(C.Structure):
 
 
 class Collider(C.Structure):
-    _fields_ = [("type", C.c_int), ("pad_", C.c_int), ("point", C.c_double * 3),
+    _fields_ = [("type", C.c_int), ("flags", C.c_int), ("point", C.c_double * 3),
                 ("normal", C.c_double * 3), ("half_extent", C.c_double * 3),
                 ("radius", C.c_double), ("velocity", C.c_double * 3), ("friction", C.c_double)]
 
@@ -150,17 +151,24 @@ def _v3(x):
     return (C.c_double * 3)(*map(float, x))
 
 
-def make_plane(point, normal, friction):
-    """make_plane (collision.hpp:14, collision.cpp:5-27)."""
-    return lib.mpm_make_plane(_v3(point), _v3(normal), float(friction))
+def _flagged(c, project):
+    if project:
+        c.flags |= PROJECT
+    return c
 
 
-def make_box(centre, half_extent, friction):
-    return lib.mpm_make_box(_v3(centre), _v3(half_extent), float(friction))
+def make_plane(point, normal, friction, project=False):
+    """make_plane (collision.hpp:14, collision.cpp:5-27).  project=True adds
+    the position projection, which the reference does not have."""
+    return _flagged(lib.mpm_make_plane(_v3(point), _v3(normal), float(friction)), project)
 
 
-def make_sphere(centre, radius, friction):
-    return lib.mpm_make_sphere(_v3(centre), float(radius), float(friction))
+def make_box(centre, half_extent, friction, project=False):
+    return _flagged(lib.mpm_make_box(_v3(centre), _v3(half_extent), float(friction)), project)
+
+
+def make_sphere(centre, radius, friction, project=False):
+    return _flagged(lib.mpm_make_sphere(_v3(centre), float(radius), float(friction)), project)
 
 
 def lame(E, nu):

@@ -105,6 +105,40 @@ def test_contact_plasticity_matches_oracle():
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
+def test_position_projection_matches_oracle():
+    """Colliders flagged for position projection (an extension; the reference
+    only changes velocities): a cube that starts 4 mm inside the ground and
+    7 mm inside a sphere cap, GPU vs oracle, and no particle left inside a
+    flagged body.  (The scene is well conditioned: a 1e-7 relative
+    perturbation of the input moves the oracle's result by 2.5e-7.  A
+    projection of many particles onto one small patch is not, so it is no
+    parity case.)"""
+    sc = _scene(0, 0.3)
+    sc["v"] = np.tile(np.array([0.5, -1.0, 0.3]), (len(sc["x"]), 1))
+    sc["x"] = sc["x"] - np.array([0, sc["x"][:, 1].min() + 0.004, 0])
+    c0 = sc["x"].mean(axis=0)
+    centre, radius = (c0[0] + 0.03, 0.004 - 0.05, c0[2] - 0.02), 0.05
+    sc["colliders"] = [snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2, project=True),
+                       snowmpm.make_sphere(centre, radius, 0.3, project=True)]
+    nsteps = 20
+    ref = mpmref.run_oracle(sc, nsteps)
+    got = mpmref.run_gpu(sc, nsteps)
+    plain = dict(sc, colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2),
+                                snowmpm.make_sphere(centre, radius, 0.3)])
+    ref_plain = mpmref.run_oracle(plain, nsteps)
+
+    def depth(x):
+        return np.minimum(x[:, 1], np.linalg.norm(x - np.array(centre), axis=1) - radius)
+
+    assert depth(ref_plain["x"]).min() < -3e-3, "without projection particles must end inside"
+    assert depth(ref["x"]).min() > 0.0
+    assert depth(got["x"]).min() > -1e-6
+    ex, ev = mpmref.rel_errors(got, ref, sc)
+    print(f"projection: rel err x {ex:.3e} v {ev:.3e}")
+    assert ex < TOL and ev < TOL
+    assert got["status"][0] == 0
+
+
 def test_track_fp_matches_oracle():
     sc = _scene(0, 0.3)
     sc["v"] = np.tile(np.array([0.0, -6.0, 0.0]), (len(sc["x"]), 1))

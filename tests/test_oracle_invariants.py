@@ -260,3 +260,32 @@ def test_box_and_sphere_sdf():
     out = o.vec(0, 0, 0)
     o.lib.orc_collision_helper(o.vec(0, -0.01, 0), o.vec(2, -3, 0), box, out)
     assert np.allclose(out[:], [1.4, 0, 0], atol=1e-12)
+
+
+def test_position_projection():
+    """The optional projection (an extension; the reference only advects,
+    grid.cpp:335-338): flagged bodies move inside particles to their closest
+    surface point (phi = 0 there), leave outside ones and velocities alone,
+    and unflagged bodies change nothing."""
+    plane = o.make_plane((0, 0.1, 0), (0, 2, 0), 0.2)
+    box = o.make_box((0, -0.5, 0), (1, 0.5, 1), 0.2)
+    sph = o.make_sphere((1, 2, 3), 0.5, 0.2)
+    for c in (plane, box, sph):
+        c.flags = 1
+    cases = [(plane, (0.3, 0.05, -0.2), (0.3, 0.1, -0.2)),     # below the plane: up to it
+             (box, (0.2, -0.1, 0.3), (0.2, 0.0, 0.3)),          # nearest face is the top
+             (box, (0.95, -0.5, 0.1), (1.0, -0.5, 0.1)),        # nearest face is x = 1
+             (sph, (1.0, 2.0, 3.2), (1.0, 2.0, 3.5))]           # radially out
+    for c, x, want in cases:
+        p = o.make_particle(x, (1.0, -2.0, 0.5), 1, 1)
+        o.position_projection(p, [c])
+        assert np.allclose(p.x[:], want, atol=1e-14), (c.type, p.x[:])
+        assert p.v[:] == [1.0, -2.0, 0.5]
+        assert abs(o.lib.orc_collider_phi(c, p.x)) < 1e-14
+    p = o.make_particle((0.3, 0.5, 0.0), (0, 0, 0), 1, 1)  # outside: unchanged
+    o.position_projection(p, [plane])
+    assert p.x[:] == [0.3, 0.5, 0.0]
+    plane.flags = 0
+    p = o.make_particle((0.3, 0.05, 0.0), (0, 0, 0), 1, 1)
+    o.position_projection(p, [plane])
+    assert p.x[:] == [0.3, 0.05, 0.0]
