@@ -676,13 +676,13 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
 
     // operands of the group at chunk position p of a cell ending at hi (the
     // all-zero record for the lanes past the end)
-    const float* lane_x = pst + kq * kP2GRec + 4 * na;
-    const float* lane_c = pst + kq * kP2GRec + kP2GRecC + 6 * nb + 2 * na;
-    const int zero_d = (kP2GChunk - kq) * kP2GRec;  // from record kq to the zero record
+    const char* lane_x = reinterpret_cast<const char*>(pst + kq * kP2GRec + 4 * na);
+    const char* lane_c = reinterpret_cast<const char*>(pst + kq * kP2GRec + kP2GRecC + 6 * nb + 2 * na);
+    const int zero_b = (kP2GChunk - kq) * kP2GRec * 4;  // bytes from record kq to the zero record
     auto fetch = [&](int p, int hi, float4& x, float2& c) {
-        const int d = p + kq < hi ? p * kP2GRec : zero_d;
-        x = *reinterpret_cast<const float4*>(lane_x + d);  // wx_a dwx_a wz_d dwz_d
-        c = *reinterpret_cast<const float2*>(lane_c + d);
+        const int b = kq < hi - p ? p * (kP2GRec * 4) : zero_b;  // hi - p and p * 160 are wave-uniform
+        x = *reinterpret_cast<const float4*>(lane_x + b);  // wx_a dwx_a wz_d dwz_d
+        c = *reinterpret_cast<const float2*>(lane_c + b);
     };
 
     for (; blk < nh; blk += G) {
@@ -721,62 +721,71 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                                           : chunk_index(hnn, blk + 2 * G < nh, 0);
             inext_ok = ok;
 
-            // cells l.. that reach into this chunk, in sorted order
-            for (; l < 64; ++l) {
-                const int cb = __builtin_amdgcn_readlane(h.cbeg, l) - c0;
-                const int ce = __builtin_amdgcn_readlane(h.cend, l) - c0;
-                if (cb >= cn) break;  // starts in a later chunk
-                const int lo = max(cb, 0), hi = min(ce, cn);
-                if (lo < hi) {
-                    if ((l >> 2) != row) {
-                        deposit_row();
-                        row = l >> 2;
+            // cells l.. that reach into this chunk, in sorted order.  The
+            // cell code is written out for each x position cx = l & 3 in the
+            // row, so its tile lands on x nodes cx..cx+3 of the row
+            // accumulators without data-dependent register selection.
+            bool more = true;
+            while (more && l < 64) {
+#pragma unroll
+                for (int cx = 0; cx < 4; ++cx) {
+                    if (!more || (l & 3) != cx) continue;
+                    const int cb = __builtin_amdgcn_readlane(h.cbeg, l) - c0;
+                    const int ce = __builtin_amdgcn_readlane(h.cend, l) - c0;
+                    if (cb >= cn) {  // starts in a later chunk
+                        more = false;
+                        continue;
                     }
-                    float4 x0, x1;
-                    float2 q0, q1;
-                    fetch(lo, hi, x0, q0);
-                    fetch(lo + 4, hi, x1, q1);
-                    __builtin_amdgcn_sched_barrier(0);  // both groups' reads in flight before the MFMAs
-                    p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
-                    auto group = [&](const float4& rx, const float2& rc) {
-                        const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
-                        const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
-                        const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
-                        const float wxy = rx.x * wy;  // wx_a wy_b
-                        cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
-                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
-                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
-                        cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
-                    };
-                    // the first two groups (the second is the zero record for a
-                    // cell piece of <= 4 particles); longer pieces loop
-                    group(x0, q0);
-                    group(x1, q1);
-                    for (int p = lo + 8; p < hi; p += 4) {
-                        fetch(p, hi, x0, q0);
+                    const int lo = max(cb, 0), hi = min(ce, cn);
+                    if (lo < hi) {
+                        if ((l >> 2) != row) {
+                            deposit_row();
+                            row = l >> 2;
+                        }
+                        float4 x0, x1;
+                        float2 q0, q1;
+                        fetch(lo, hi, x0, q0);
+                        fetch(lo + 4, hi, x1, q1);
+                        __builtin_amdgcn_sched_barrier(0);  // both groups' reads in flight before the MFMAs
+                        // even cx: the MFMAs accumulate straight into x nodes cx..cx+3
+                        // of the row (an even-aligned register quad, as MFMA operands
+                        // must be); odd cx: a fresh tile, added afterwards
+                        p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
+                        if ((cx & 1) == 0) {
+                            cm = p2g_v4f{am[cx], am[cx + 1], am[cx + 2], am[cx + 3]};
+                            cf = p2g_v4f{af[cx], af[cx + 1], af[cx + 2], af[cx + 3]};
+                        }
+                        auto group = [&](const float4& rx, const float2& rc) {
+                            const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
+                            const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
+                            const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
+                            const float wxy = rx.x * wy;  // wx_a wy_b
+                            cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
+                        };
+                        // the first two groups (the second is the zero record for a
+                        // cell piece of <= 4 particles); longer pieces loop
                         group(x0, q0);
+                        group(x1, q1);
+                        for (int p = lo + 8; p < hi; p += 4) {
+                            fetch(p, hi, x0, q0);
+                            group(x0, q0);
+                        }
+                        if ((cx & 1) == 0) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r + cx] = cm[r], af[r + cx] = cf[r];
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) am[r + cx] += cm[r], af[r + cx] += cf[r];
+                        }
                     }
-                    // x nodes cx + r of the row
-                    switch (l & 3) {
-                        case 0:
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r] += cm[r], af[r] += cf[r];
-                            break;
-                        case 1:
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r + 1] += cm[r], af[r + 1] += cf[r];
-                            break;
-                        case 2:
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r + 2] += cm[r], af[r + 2] += cf[r];
-                            break;
-                        default:
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r + 3] += cm[r], af[r + 3] += cf[r];
-                            break;
-                    }
+                    if (ce > cn)  // continues in the next chunk
+                        more = false;
+                    else
+                        ++l;
                 }
-                if (ce > cn) break;  // continues in the next chunk
             }
         }
         deposit_row();
