@@ -92,7 +92,7 @@ typedef struct {
 } mpm_timings;
 
 typedef struct {
-    size_t particle_capacity; /* total particles of all resident scenes */
+    size_t particle_capacity; /* total particles of all resident scenes, < 2^30 */
     int hash_log2;            /* block hash slots = 2^hash_log2; 0 = sized from capacity */
     int track_fp;             /* 1: keep the full plastic gradient F_P; 0: keep J_P = det F_P */
     int device;               /* HIP device ordinal */

@@ -74,6 +74,7 @@ struct PBuf {
     float* S[kStressCh];  // force matrix for the next P2G: S00 S11 S22 S01 S02 S12
     uint32_t* pid; // particle index within its scene (download order, sort tie-break)
     uint32_t* sid; // scene id
+    uint32_t stride;  // elements from one component of x, v, F, Fp, S to the next
 };
 
 // The fields a step changes (x v F_E J_P F_P S) and the ones it does not

@@ -59,8 +59,9 @@ int Engine::dalloc(void** p, size_t bytes) {
 
 int Engine::init(const mpm_config& cfg) {
     cfg_ = cfg;
-    if (cfg_.particle_capacity == 0 || cfg_.particle_capacity > 0x7fffffffull) {
-        set_error("particle_capacity must be in [1, 2^31)");
+    // Kernels address particle arrays with 32-bit byte offsets (index * 4).
+    if (cfg_.particle_capacity == 0 || cfg_.particle_capacity >= (1ull << 30)) {
+        set_error("particle_capacity must be in [1, 2^30)");
         return MPM_ERR_ARG;
     }
     if (cfg_.kappa <= 0) cfg_.kappa = 1.5;
@@ -113,6 +114,7 @@ int Engine::init(const mpm_config& cfg) {
         p.V0 = take();
         p.E = take();
         for (int e = 0; e < 6; ++e) p.S[e] = take();
+        p.stride = (uint32_t)N;  // components of x, v, F, Fp and S lie N elements apart
         if ((rc = dalloc((void**)&p.pid, N * 4))) return rc;
         if ((rc = dalloc((void**)&p.sid, N * 4))) return rc;
     }

@@ -159,6 +159,21 @@ __device__ __forceinline__ int bin_leader32(uint32_t v, bool want, uint64_t& gro
     return leader;
 }
 
+// Element i of a particle array as a 32-bit byte offset from the array's
+// wave-uniform base: the access takes the base from scalar registers and needs
+// no 64-bit address arithmetic per lane.  Particle indices are below 2^30
+// (the engine's capacity limit), so the offset does not wrap.
+template <typename T>
+__device__ __forceinline__ T& at(T* p, uint32_t i) {
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (i << 2));
+}
+// Component e of a vector field (x, v, F, Fp, S) of particle i: components
+// are p.stride elements apart, so a kernel needs only the first one's base.
+__device__ __forceinline__ float& comp(float* const* f, const PBuf& p, int e, uint32_t i) {
+    return at(f[0] + (size_t)e * p.stride, i);
+}
+
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
     if (threadIdx.x == 0 && blockIdx.x == 0) {  // per-pass counters (no kernel of this pass has read them yet)
@@ -179,9 +194,9 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
             const uint32_t i = base + 64 * s + lane;
             if (i < n) {
                 nsid[s] = a.cur.sid[i];
-                nx[s][0] = a.cur.x[0][i];
-                nx[s][1] = a.cur.x[1][i];
-                nx[s][2] = a.cur.x[2][i];
+                nx[s][0] = comp(a.cur.x, a.cur, 0, i);
+                nx[s][1] = comp(a.cur.x, a.cur, 1, i);
+                nx[s][2] = comp(a.cur.x, a.cur, 2, i);
             }
         }
     };
@@ -578,12 +593,12 @@ struct P2GRaw {
 __device__ __forceinline__ void p2g_load(const StepArgs& a, uint32_t i, P2GRaw& r) {
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
-        r.x[e] = a.cur.x[e][i];
-        r.v[e] = a.cur.v[e][i];
+        r.x[e] = comp(a.cur.x, a.cur, e, i);
+        r.v[e] = comp(a.cur.v, a.cur, e, i);
     }
-    r.m = a.cur.m[i];
+    r.m = at(a.cur.m, i);
 #pragma unroll
-    for (int e = 0; e < kStressCh; ++e) r.S[e] = a.cur.S[e][i];
+    for (int e = 0; e < kStressCh; ++e) r.S[e] = comp(a.cur.S, a.cur, e, i);
 }
 
 // Stage a particle as record rec: weights of the three axes, m, m v, S / dx.
@@ -1142,7 +1157,7 @@ __device__ __forceinline__ void elastic_stress(const Sym& E, float J, float mu, 
 
 __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
     if (sc.use_E) {
-        const float E = p.E[i];
+        const float E = at(p.E, i);
         mu0 = E * sc.mu_per_E;
         la0 = E * sc.la_per_E;
     } else {
@@ -1228,9 +1243,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
         const float inv_dx = sc.inv_dx, dt = sc.dt, alpha = sc.alpha;
         for (uint32_t j = off + threadIdx.x; j < off + cnt; j += 256) {
-            const uint32_t i = a.bin.sidx[j];
-            const float x0 = a.cur.x[0][i], x1 = a.cur.x[1][i], x2 = a.cur.x[2][i];
-            float v0 = a.cur.v[0][i], v1 = a.cur.v[1][i], v2 = a.cur.v[2][i];
+            const uint32_t i = at(a.bin.sidx, j);
+            const float x0 = comp(a.cur.x, a.cur, 0, i), x1 = comp(a.cur.x, a.cur, 1, i), x2 = comp(a.cur.x, a.cur, 2, i);
+            float v0 = comp(a.cur.v, a.cur, 0, i), v1 = comp(a.cur.v, a.cur, 1, i), v2 = comp(a.cur.v, a.cur, 2, i);
             int b[3];
             float f[3];
             base_node(sc, x0, x1, x2, b, f);
@@ -1245,9 +1260,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             // hides under the gather.
             float Fe[9];
 #pragma unroll
-            for (int e = 0; e < 9; ++e) Fe[e] = a.cur.F[e][i];
-            const float Jp = a.cur.Jp[i];
-            const float V0 = a.cur.V0[i];
+            for (int e = 0; e < 9; ++e) Fe[e] = comp(a.cur.F, a.cur, e, i);
+            const float Jp = at(a.cur.Jp, i);
+            const float V0 = at(a.cur.V0, i);
             float mu0, la0;
             lame(sc, a.cur, i, mu0, la0);
             float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
@@ -1302,7 +1317,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             float Fp_old[9], Fp_new[9];
             constexpr bool track = TRACK;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) Fp_old[e] = TRACK ? a.cur.Fp[e][i] : 0.f;
+            for (int e = 0; e < 9; ++e) Fp_old[e] = TRACK ? comp(a.cur.Fp, a.cur, e, i) : 0.f;
             PlastOut po;
             float Fh[9];
 #pragma unroll
@@ -1353,14 +1368,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             const PBuf& o = a.nxt;
             const uint32_t w = PERMUTE ? j : i;
 #pragma unroll
-            for (int e = 0; e < 9; ++e) o.F[e][w] = po.Fe[e];
-            o.Jp[w] = po.Jp;
+            for (int e = 0; e < 9; ++e) comp(o.F, o, e, w) = po.Fe[e];
+            at(o.Jp, w) = po.Jp;
             if (track) {
 #pragma unroll
-                for (int e = 0; e < 9; ++e) o.Fp[e][w] = Fp_new[e];
+                for (int e = 0; e < 9; ++e) comp(o.Fp, o, e, w) = Fp_new[e];
             }
 #pragma unroll
-            for (int e = 0; e < 6; ++e) o.S[e][w] = po.S[e];
+            for (int e = 0; e < 6; ++e) comp(o.S, o, e, w) = po.S[e];
 
             // v_p = (1-alpha) v_pic + alpha (v_p + sum w (v_i - p_i/m_i))  (grid.cpp:321-324)
             v0 = alpha * v0 + pv0;
@@ -1371,18 +1386,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
             if (PROJECT && sc.nproject) project<float>(sc, nx0, nx1, nx2);
             if (!finite3(nx0, nx1, nx2) || !finite3(v0, v1, v2) || !isfinite(po.Jp)) flag(a.st, sid, kStatusNaN);
 
-            o.x[0][w] = nx0;
-            o.x[1][w] = nx1;
-            o.x[2][w] = nx2;
-            o.v[0][w] = v0;
-            o.v[1][w] = v1;
-            o.v[2][w] = v2;
+            comp(o.x, o, 0, w) = nx0;
+            comp(o.x, o, 1, w) = nx1;
+            comp(o.x, o, 2, w) = nx2;
+            comp(o.v, o, 0, w) = v0;
+            comp(o.v, o, 1, w) = v1;
+            comp(o.v, o, 2, w) = v2;
             if (PERMUTE) {
-                a.nxt.m[j] = a.cur.m[i];
-                a.nxt.V0[j] = V0;
-                if (a.copy_E) a.nxt.E[j] = a.cur.E[i];
-                a.nxt.pid[j] = a.cur.pid[i];
-                a.nxt.sid[j] = sid;
+                at(a.nxt.m, j) = at(a.cur.m, i);
+                at(a.nxt.V0, j) = V0;
+                if (a.copy_E) at(a.nxt.E, j) = at(a.cur.E, i);
+                at(a.nxt.pid, j) = at(a.cur.pid, i);
+                at(a.nxt.sid, j) = sid;
             }
         }
         __syncthreads();
