@@ -93,6 +93,10 @@ int Engine::init(const mpm_config& cfg) {
     H_ = 1u << hl;
     launch_.grid_l = (int)std::max<uint32_t>(1u, H_ / 4096u);
     launch_.grid_k = cus * 2;
+    // P2G: enough single-wave workgroups for every home block of a scene
+    // with >= 256 particles per occupied block (a dense 8-per-cell block has
+    // 512); more home blocks than that fall back to the persistent loop.
+    launch_.grid_h = (int)std::min<size_t>((N + 255) / 256, (size_t)1 << 20);
     const bool track = cfg_.track_fp != 0;
 
     int rc;

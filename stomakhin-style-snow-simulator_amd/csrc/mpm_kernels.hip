@@ -1600,6 +1600,13 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
 // ---------------------------------------------------------------------------
 // Host launch wrappers.
 
+// P2G runs one single-wave workgroup per home block when the grid covers
+// them all: the dispatcher then balances the blocks across the CUs, which a
+// persistent loop over blk, blk + G, ... does not (a workgroup that starts
+// late and takes a second block becomes the tail).  Workgroups past the last
+// home block exit at once.
+inline int p2g_grid(const LaunchCfg& c) { return c.grid_w * 4 > c.grid_h ? c.grid_w * 4 : c.grid_h; }
+
 // G2P with or without the position projection (a branch per particle only
 // when some scene asks for it).
 template <bool TRACK, bool PERMUTE>
@@ -1618,7 +1625,7 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[1], s);
-    k_p2g<<<c.grid_w * 4, 64, 0, s>>>(a);
+    k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[2], s);
     k_grid<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
@@ -1644,7 +1651,7 @@ void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
-    k_p2g<<<c.grid_w * 4, 64, 0, s>>>(a);
+    k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
     StepArgs am = a;
     am.mass_only = 1;
     k_grid<<<c.grid_b, 256, 0, s>>>(am);

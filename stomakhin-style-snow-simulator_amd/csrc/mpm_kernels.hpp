@@ -32,6 +32,7 @@ struct LaunchCfg {
     int grid_w;  // one workgroup per block kernels (P2G, G2P)
     int grid_l;  // hash-slot compaction (4096 slots per workgroup)
     int grid_k;  // k_blocks (16 blocks per workgroup)
+    int grid_h;  // lower bound of the P2G grid (one wave per home block)
 };
 
 // Kernel slots timed by launch_step's events:
