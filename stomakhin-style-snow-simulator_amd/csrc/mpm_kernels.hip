@@ -1606,6 +1606,9 @@ __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, do
 // late and takes a second block becomes the tail).  Workgroups past the last
 // home block exit at once.
 inline int p2g_grid(const LaunchCfg& c) { return c.grid_w * 4 > c.grid_h ? c.grid_w * 4 : c.grid_h; }
+// The grid kernel likewise gets a wave for every block (home blocks and
+// their neighbours, about 1.4x the home blocks for a dense slab).
+inline int grid_grid(const LaunchCfg& c) { return c.grid_b > c.grid_h / 2 ? c.grid_b : c.grid_h / 2; }
 
 // G2P with or without the position projection (a branch per particle only
 // when some scene asks for it).
@@ -1627,7 +1630,7 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     if (ev) (void)hipEventRecord(ev[1], s);
     k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[2], s);
-    k_grid<<<c.grid_b, 256, 0, s>>>(a);
+    k_grid<<<grid_grid(c), 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
     if (a.cur.Fp[0]) {
         if (a.permute)
@@ -1654,7 +1657,7 @@ void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
     StepArgs am = a;
     am.mass_only = 1;
-    k_grid<<<c.grid_b, 256, 0, s>>>(am);
+    k_grid<<<grid_grid(c), 256, 0, s>>>(am);
     k_setup_gather<<<c.grid_w, 256, 0, s>>>(a);
     k_end<<<c.grid_b, 256, 0, s>>>(a, 0);
 }
