@@ -52,9 +52,13 @@ def test_freefall_closed_form():
     assert np.abs(got["Fe"] - np.eye(3)).max() < 1e-5
 
 
-@pytest.mark.parametrize("kind,scale", [(0, 0.5), (1, 0.15)])
-def test_ten_steps_match_oracle(kind, scale):
-    sc = _scene(kind, scale)
+@pytest.mark.parametrize("kind,scale,index", [(0, 0.5, 0), (1, 0.15, 0), (2, 0.05, 0), (3, 0.1, 0), (3, 0.1, 1),
+                                               (3, 0.1, 2)])
+def test_ten_steps_match_oracle(kind, scale, index):
+    """Each workload at reduced size: the cube, the two tori, the slab (per-particle
+    E, box collider) and scenes of the batched distribution (box, sphere and
+    torus shapes, random velocities up to 8 m/s)."""
+    sc = _scene(kind, scale, index=index)
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
     ex, ev = mpmref.rel_errors(got, ref, sc)
@@ -101,6 +105,24 @@ def test_contact_plasticity_matches_oracle():
     assert (jp_ref < 0.999).any(), "scenario must plastify"
     ex, ev = mpmref.rel_errors(got, ref, sc)
     print(f"contact: rel err x {ex:.3e} v {ev:.3e}, min Jp {jp_ref.min():.4f}")
+    assert ex < TOL and ev < 1e-3
+    assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
+
+
+def test_slab_contact_matches_oracle():
+    """The slab workload (per-particle E, box collider with friction 0.2)
+    driven into the box top: contact, friction and plasticity against the
+    oracle."""
+    sc = _scene(2, 0.05)
+    sc["x"] = sc["x"] - np.array([0, sc["x"][:, 1].min() - 0.005, 0])
+    sc["v"] = np.tile(np.array([0.4, -4.0, -0.3]), (len(sc["x"]), 1))
+    nsteps = 30
+    ref = mpmref.run_oracle(sc, nsteps)
+    got = mpmref.run_gpu(sc, nsteps)
+    jp_ref = np.linalg.det(ref["Fp"])
+    assert (jp_ref < 0.999).any(), "scenario must plastify"
+    ex, ev = mpmref.rel_errors(got, ref, sc)
+    print(f"slab contact: rel err x {ex:.3e} v {ev:.3e}, min Jp {jp_ref.min():.4f}")
     assert ex < TOL and ev < 1e-3
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
