@@ -440,6 +440,7 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
                 I[t] = a.bin.sidx[off + t];
             }
             __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
+            bool moved = false;
             for (uint32_t k = 1; k < cnt; ++k) {
                 const uint32_t kpk = P[c0 + k], kik = I[c0 + k];
                 uint32_t j = k;
@@ -450,11 +451,17 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
                 }
                 P[c0 + j] = kpk;
                 I[c0 + j] = kik;
+                moved |= j != k;
             }
             __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
-            for (uint32_t t = lane; t < nb; t += 64) {
-                a.bin.spid[off + t] = P[t];
-                a.bin.sidx[off + t] = I[t];
+            // Most blocks arrive sorted (the buffer keeps the previous order
+            // and few particles change cells): only reordered blocks are
+            // written back.
+            if (__ballot(moved)) {
+                for (uint32_t t = lane; t < nb; t += 64) {
+                    a.bin.spid[off + t] = P[t];
+                    a.bin.sidx[off + t] = I[t];
+                }
             }
             __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
         } else {
