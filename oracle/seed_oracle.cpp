@@ -24,7 +24,8 @@
 // Blender's BVH ray caster agrees bit for bit on points within rounding of
 // the surface cannot be checked here (bpy is not available): parity with
 // Blender is unpinned for such points; the tests pin this restatement by
-// closed forms (boxes, convex polyhedra, a box with a cavity).
+// closed forms (boxes, convex polyhedra, a box with a cavity) and the lattice
+// by the add-on's own output shipped as the reference's input/particles.csv.
 //
 // Compiled without floating-point contraction (-ffp-contract=off): every
 // product and sum is rounded on its own, as the GPU kernel does.
