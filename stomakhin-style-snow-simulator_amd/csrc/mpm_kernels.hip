@@ -787,10 +787,11 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                             cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
                             cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
                         };
-                        // the first two groups (the second is the zero record for a
-                        // cell piece of <= 4 particles); longer pieces loop
+                        // the first two groups (a cell piece of <= 4 particles needs
+                        // only the first: the second would read the zero record);
+                        // longer pieces loop
                         group(x0, q0);
-                        group(x1, q1);
+                        if (hi - lo > 4) group(x1, q1);
                         for (int p = lo + 8; p < hi; p += 4) {
                             fetch(p, hi, x0, q0);
                             group(x0, q0);
