@@ -1176,10 +1176,15 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 
 // ---------------------------------------------------------------------------
 // G2P grid staging: neighbour block t of a home block (t = 0..7) is staged by
-// the 32 threads 32 t .. 32 t + 31, each holding 12 of its 6 x 64 floats, so
-// one slot lookup per thread suffices and the loads are 128-byte runs.
+// a group of 32 threads, each holding 12 of its 6 x 64 floats, so one slot
+// lookup per thread and block suffices and the loads are 128-byte runs.
 constexpr int kG2PStage = kVelCh * 64 / 32;
 static_assert(kG2PStage * 32 == kVelCh * 64, "staging split");
+// Threads per G2P workgroup: one workgroup per home block, so a block's
+// particles take ceil(count / T) passes; 128 leaves fewer idle lanes in the
+// last pass than 256 for the 100-500-particle blocks of uneven scenes.
+constexpr int kG2PThreads = 128;
+static_assert(kG2PThreads % 32 == 0 && 8 % (kG2PThreads / 32) == 0, "neighbour blocks per 32-thread group");
 
 __device__ __forceinline__ void g2p_stage_load(const StepArgs& a, uint32_t ts, int l32, float (&r)[kG2PStage]) {
     const float* src = a.g.vu + (size_t)ts * kVelCh * 64;
@@ -1206,51 +1211,43 @@ __device__ __forceinline__ void g2p_stage_store(float4* va, float2* ub, int tg, 
 }
 
 // Fused stress_update + G2P + particle collisions + advection.  One workgroup
-// per home block stages the 8x8x8 nodes of its 8 blocks (v and u) in LDS,
+// per home block (the grid covers the home blocks, so the dispatcher balances
+// them over the CUs) stages the 8x8x8 nodes of its 8 blocks (v and u) in LDS,
 // then each thread updates one particle and writes it to the next buffer at
 // its sorted position.
 // Four waves per SIMD: the register budget (<= 128 VGPRs) is met without
-// spills and the extra resident waves hide the particle and grid loads.
+// spills, and eight workgroups per CU hide each other's staging and particle
+// loads.
 template <bool TRACK, bool PERMUTE, bool PROJECT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
+__global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
     __shared__ float4 va[512];  // [Z][Y][X]: v0 v1 v2 u0
     __shared__ float2 ub[512];  // [Z][Y][X]: u1 u2
     const uint32_t nh = a.cnt->n_home;
-    // Software pipeline over this workgroup's blocks blk, blk + G, blk + 2G:
-    // while block blk is computed, the grid data of blk + G is in flight into
-    // registers and the neighbour slots of blk + 2G are being looked up, so the
-    // dependent loads home_list -> nb -> vu never stall the block start.
-    // Threads 32 tg .. 32 tg + 31 stage neighbour block tg (12 floats each).
+    // Threads 32 tg .. 32 tg + 31 stage neighbour blocks tg, tg + T/32, ...
+    // (T threads per workgroup).
+    constexpr int kGroups = kG2PThreads / 32;
     const uint32_t G = gridDim.x;
     const int tg = (int)(threadIdx.x >> 5), l32 = (int)(threadIdx.x & 31);
-    uint32_t blk = blockIdx.x;
-    float stg[kG2PStage];
-    uint32_t slot = 0, slot1 = 0, ts1 = 0;
-    if (blk < nh) {
-        slot = a.g.home_list[blk];
-        g2p_stage_load(a, a.g.nb[(size_t)slot * 8 + tg], l32, stg);
-    }
-    if (blk + G < nh) {
-        slot1 = a.g.home_list[blk + G];
-        ts1 = a.g.nb[(size_t)slot1 * 8 + tg];
-    }
-    for (; blk < nh; blk += G) {
-        g2p_stage_store(va, ub, tg, l32, stg);
+    for (uint32_t blk = blockIdx.x; blk < nh; blk += G) {
+        const uint32_t slot = a.g.home_list[blk];
+        {
+            float stg[8 / kGroups][kG2PStage];
+#pragma unroll
+            for (int k = 0; k < 8 / kGroups; ++k)
+                g2p_stage_load(a, a.g.nb[(size_t)slot * 8 + tg + k * kGroups], l32, stg[k]);
+            __syncthreads();  // the previous block's gathers are done
+#pragma unroll
+            for (int k = 0; k < 8 / kGroups; ++k) g2p_stage_store(va, ub, tg + k * kGroups, l32, stg[k]);
+        }
         uint32_t sid;
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
         const SceneDev& sc = a.scenes[sid];
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
-        uint32_t slot2 = 0, ts2 = 0;
-        if (blk + G < nh) g2p_stage_load(a, ts1, l32, stg);
-        if (blk + 2 * G < nh) {
-            slot2 = a.g.home_list[blk + 2 * G];
-            ts2 = a.g.nb[(size_t)slot2 * 8 + tg];
-        }
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
         const float inv_dx = sc.inv_dx, dt = sc.dt, alpha = sc.alpha;
-        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += 256) {
+        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += kG2PThreads) {
             const uint32_t i = at(a.bin.sidx, j);
             const float x0 = comp(a.cur.x, a.cur, 0, i), x1 = comp(a.cur.x, a.cur, 1, i), x2 = comp(a.cur.x, a.cur, 2, i);
             float v0 = comp(a.cur.v, a.cur, 0, i), v1 = comp(a.cur.v, a.cur, 1, i), v2 = comp(a.cur.v, a.cur, 2, i);
@@ -1408,10 +1405,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                 at(a.nxt.sid, j) = sid;
             }
         }
-        __syncthreads();
-        slot = slot1;
-        slot1 = slot2;
-        ts1 = ts2;
     }
 }
 
@@ -1617,15 +1610,17 @@ inline int p2g_grid(const LaunchCfg& c) { return c.grid_w * 4 > c.grid_h ? c.gri
 // The grid kernel likewise gets a wave for every block (home blocks and
 // their neighbours, about 1.4x the home blocks for a dense slab).
 inline int grid_grid(const LaunchCfg& c) { return c.grid_b > c.grid_h / 2 ? c.grid_b : c.grid_h / 2; }
+// G2P: a workgroup per home block as well.
+inline int g2p_grid(const LaunchCfg& c) { return c.grid_w > c.grid_h ? c.grid_w : c.grid_h; }
 
 // G2P with or without the position projection (a branch per particle only
 // when some scene asks for it).
 template <bool TRACK, bool PERMUTE>
 void launch_g2p(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     if (a.project)
-        k_g2p<TRACK, PERMUTE, true><<<c.grid_w, 256, 0, s>>>(a);
+        k_g2p<TRACK, PERMUTE, true><<<g2p_grid(c), kG2PThreads, 0, s>>>(a);
     else
-        k_g2p<TRACK, PERMUTE, false><<<c.grid_w, 256, 0, s>>>(a);
+        k_g2p<TRACK, PERMUTE, false><<<g2p_grid(c), kG2PThreads, 0, s>>>(a);
 }
 
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
