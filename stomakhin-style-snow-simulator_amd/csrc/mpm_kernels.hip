@@ -661,16 +661,6 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     if (blk >= nh) return;
     for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < kP2GRec + 2) pst[kP2GChunk * kP2GRec + lane] = 0.0f;
-    // This lane's flush outputs o = lane + 64 q and their region indices,
-    // two 16-bit indices per register, loaded once.
-    uint32_t ftab[kP2GFlushQ / 2];
-#pragma unroll
-    for (int q = 0; q < kP2GFlushQ; q += 2) {
-        const int o0 = lane + 64 * q, o1 = o0 + 64;
-        const uint32_t e0 = o0 < kGridCh * kRegion ? kFlush.e[o0] : 0u;
-        const uint32_t e1 = o1 < kGridCh * kRegion ? kFlush.e[o1] : 0u;
-        ftab[q / 2] = e0 | (e1 << 16);
-    }
 
     // Chunk pipeline over this wave's blocks blk, blk + G, ...: while chunk k
     // is accumulated, the fields of chunk k + 1 and the sorted indices of
@@ -819,6 +809,16 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         // region for the next block.
         const uint32_t slot3 = blk + 3 * G < nh ? a.g.home_list[blk + 3 * G] : 0u;
         if (blk < a.g.max_home) {
+            // This lane's flush outputs o = lane + 64 q and their region
+            // indices, two 16-bit indices per register.
+            uint32_t ftab[kP2GFlushQ / 2];
+#pragma unroll
+            for (int q = 0; q < kP2GFlushQ; q += 2) {
+                const int o0 = lane + 64 * q, o1 = o0 + 64;
+                const uint32_t e0 = o0 < kGridCh * kRegion ? kFlush.e[o0] : 0u;
+                const uint32_t e1 = o1 < kGridCh * kRegion ? kFlush.e[o1] : 0u;
+                ftab[q / 2] = e0 | (e1 << 16);
+            }
             float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
 #pragma unroll
             for (int q = 0; q < kP2GFlushQ; ++q) {
@@ -829,7 +829,8 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
             flag(a.st, sid, kStatusOverflow);
         }
         __asm__ __volatile__("" ::: "memory");  // region reads before the clearing writes
-        for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (blk + G < nh)  // another block follows in this wave
+            for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         h = hn;
         hn = hnn;
         if (blk + 3 * G < nh) p2g_header(a, slot3, lane, hnn);
