@@ -17,6 +17,7 @@ The seeding entry (mesh -> lattice particles, the reference's Blender add-on
 restated on the GPU) is reported beside them with its own CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving] [--no-seeding]
+       python bench.py --batched-only --steps K   (the batched form alone, for profiling)
 """
 import argparse
 import json
@@ -79,7 +80,7 @@ def copy_calibration(nbytes=1 << 30):
 
 
 def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(10, 25, 50, 100, 200),
-                       requests_per_rate=64):
+                       requests_per_rate=64, batched_only=False):
     """Batched form (256 uneven scenes in one engine, graph-replayed step,
     autotuned launch shapes) and served form (Poisson arrivals, continuous
     batching) against the plain per-scene GPU step one scene at a time.
@@ -106,6 +107,8 @@ def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(1
     out["batched_scenes_per_s"] = n_scenes / wall
     out["batched_ms_per_step"] = wall * 1e3 / steps
     out["batched_particle_steps_per_s"] = total * steps / wall
+    if batched_only:
+        return out
     # plain: one scene at a time, plain launches (sampled)
     idx = np.linspace(0, n_scenes - 1, plain_sample).astype(int)
     eng = snowmpm.Engine(capacity=int(ns.max()) + 4096)
@@ -220,6 +223,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true", help="skip the batched / served measurements")
     ap.add_argument("--no-seeding", action="store_true", help="skip the mesh-seeding measurement")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the batched form (256 scenes), --steps graph-replayed steps each (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -230,6 +235,9 @@ def main():
 
     import snowmpm
 
+    if args.batched_only:
+        print(json.dumps({"batched": batched_and_served(args.seed, steps=args.steps, batched_only=True)}))
+        return
     t0 = time.perf_counter()
     sc = snowmpm.generate_scene(2, seed=args.seed, scale=1.0)
     n = sc["x"].shape[0]
