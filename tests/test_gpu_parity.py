@@ -109,6 +109,25 @@ def test_contact_plasticity_matches_oracle():
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
+def test_sparse_blocks_exceed_the_grid():
+    """21,952 particles 0.1 m apart, each in its own home block: more home
+    blocks than the P2G and G2P grids hold workgroups (the persistent-loop
+    path), random velocities and a ground plane, against the oracle."""
+    rng = np.random.default_rng(8)
+    g = np.stack(np.meshgrid(*[np.arange(28)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = g * 0.1 + np.array([-1.4, 0.05, -1.4]) + rng.uniform(-0.01, 0.01, (len(g), 3))
+    v = rng.uniform(-3, 3, x.shape)
+    prm = snowmpm.params(1e-4, 0.02, *snowmpm.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    sc = dict(x=x, v=v, m=np.full(len(x), 4e-4), params=prm,
+              colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
+    ref = mpmref.run_oracle(sc, 5)
+    got = mpmref.run_gpu(sc, 5)
+    ex, ev = mpmref.rel_errors(got, ref, sc)
+    print(f"sparse: {len(x)} particles, rel err x {ex:.3e} v {ev:.3e}")
+    assert ex < TOL and ev < TOL
+    assert got["status"][0] == 0
+
+
 def test_slab_contact_matches_oracle():
     """The slab workload (per-particle E, box collider with friction 0.2)
     driven into the box top: contact, friction and plasticity against the
