@@ -258,6 +258,9 @@ def main():
     t_down = time.perf_counter()
     eng.download(sid)
     t_down = time.perf_counter() - t_down
+    t_down_x = time.perf_counter()
+    eng.download(sid, fields=("x",))
+    t_down_x = time.perf_counter() - t_down_x
     eng.close()
 
     ms_step = wall * 1e3 / args.steps
@@ -293,11 +296,12 @@ def main():
         "status": st,
         "fp64_fallbacks": fb,
         # Host buffers crossing PCIe (not part of value): scene upload from double
-        # host arrays, state download to double host arrays, and the rate when
-        # every 80-step frame is read back, as the reference driver does.
-        "host_io": {"upload_s": t_up, "download_s": t_down,
+        # host arrays, download of the whole state and of the positions alone to
+        # double host arrays, and the rate when the positions of every 80-step
+        # frame are read back, as the reference driver does to export them.
+        "host_io": {"upload_s": t_up, "download_s": t_down, "download_x_s": t_down_x,
                     "particle_steps_per_s_with_frame_download":
-                        n * 80 / (80 * ms_step * 1e-3 + t_down)},
+                        n * 80 / (80 * ms_step * 1e-3 + t_down_x)},
     }
     if not args.no_serving:
         out["batched"] = batched_and_served(args.seed)

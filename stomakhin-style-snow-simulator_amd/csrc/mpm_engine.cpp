@@ -237,52 +237,30 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
                   std::to_string(cfg_.particle_capacity));
         return MPM_ERR_CAPACITY;
     }
-    const bool track = cfg_.track_fp != 0;
-    // Host staging: SoA float fields in device-buffer order.
-    const int nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1;
-    std::vector<float> stage((size_t)nf * n);
-    std::vector<uint32_t> pid(n), sidv(n, (uint32_t)sid);
-    for (size_t i = 0; i < n; ++i) {
-        size_t o = 0;
-        for (int e = 0; e < 3; ++e, o += n) stage[o + i] = (float)x[i * 3 + e];
-        for (int e = 0; e < 3; ++e, o += n) stage[o + i] = (float)v[i * 3 + e];
-        double F[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
-        if (Fe) std::memcpy(F, Fe + i * 9, sizeof F);
-        if (Fp) std::memcpy(P, Fp + i * 9, sizeof P);
-        for (int e = 0; e < 9; ++e, o += n) stage[o + i] = (float)F[e];
-        const double jp = P[0] * (P[4] * P[8] - P[5] * P[7]) - P[1] * (P[3] * P[8] - P[5] * P[6]) +
-                          P[2] * (P[3] * P[7] - P[4] * P[6]);
-        stage[o + i] = (float)jp;
-        o += n;
-        if (track)
-            for (int e = 0; e < 9; ++e, o += n) stage[o + i] = (float)P[e];
-        stage[o + i] = (float)m[i];
-        o += n;
-        stage[o + i] = V0 ? (float)V0[i] : 0.0f;
-        o += n;
-        stage[o + i] = E ? (float)E[i] : 0.0f;
-        pid[i] = (uint32_t)i;
-    }
-    const PBuf p = view(mov_, con_);
+    // The caller's double arrays go to the GPU as they are; a kernel converts
+    // them into the FP32 buffers at particles n0 .. n0 + n - 1.
+    const double* const host[7] = {x, v, m, V0, Fe, Fp, E};
+    const size_t per[7] = {3, 3, 1, 1, 9, 9, 1};
+    size_t total = 0;
+    for (int k = 0; k < 7; ++k)
+        if (host[k]) total += per[k] * n;
+    double* d_in = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_in, total * sizeof(double), stream_));
+    const double* dev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t off = 0;
+    for (int k = 0; k < 7; ++k)
+        if (host[k]) {
+            HIP_TRY(hipMemcpyAsync(d_in + off, host[k], per[k] * n * sizeof(double), hipMemcpyHostToDevice, stream_));
+            dev[k] = d_in + off;
+            off += per[k] * n;
+        }
     {
-        size_t o = 0;
-        auto put = [&](float* dst) -> hipError_t {
-            hipError_t e = hipMemcpyAsync(dst + n0, stage.data() + o, n * 4, hipMemcpyHostToDevice, stream_);
-            o += n;
-            return e;
-        };
-        for (int e = 0; e < 3; ++e) HIP_TRY(put(p.x[e]));
-        for (int e = 0; e < 3; ++e) HIP_TRY(put(p.v[e]));
-        for (int e = 0; e < 9; ++e) HIP_TRY(put(p.F[e]));
-        HIP_TRY(put(p.Jp));
-        if (track)
-            for (int e = 0; e < 9; ++e) HIP_TRY(put(p.Fp[e]));
-        HIP_TRY(put(p.m));
-        HIP_TRY(put(p.V0));
-        HIP_TRY(put(p.E));
-        HIP_TRY(hipMemcpyAsync(p.pid + n0, pid.data(), n * 4, hipMemcpyHostToDevice, stream_));
-        HIP_TRY(hipMemcpyAsync(p.sid + n0, sidv.data(), n * 4, hipMemcpyHostToDevice, stream_));
+        const StepArgs a = args(kSceneActive);
+        launch_put_scene(a, launch_, n0, (uint32_t)n, (uint32_t)sid, SceneIn{dev[0], dev[1], dev[2], dev[3], dev[4], dev[5], dev[6]},
+                         stream_);
+        HIP_TRY(hipGetLastError());
     }
+    HIP_TRY(hipFreeAsync(d_in, stream_));
     SceneDev& sd = h_scenes_[sid];
     std::memset(&sd, 0, sizeof sd);
     sd.dx = (float)prm.dx;
@@ -458,31 +436,30 @@ int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp
         return MPM_ERR_STATE;
     }
     const size_t n = scenes_[sid].n;
-    const int nf = 17 + (track ? 9 : 0);
-    float* d_out = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&d_out, (size_t)nf * n * 4, stream_));
+    // The GPU writes the requested arrays in the caller's layout (doubles,
+    // original particle order); the host only copies them.
+    double* const host[6] = {x, v, Fe, Fp, Jp, V0};
+    const size_t per[6] = {3, 3, 9, 9, 1, 1};
+    size_t total = 0;
+    for (int k = 0; k < 6; ++k)
+        if (host[k]) total += per[k] * n;
+    if (total == 0 || n == 0) return MPM_OK;
+    double* d_out = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d_out, total * sizeof(double), stream_));
+    double* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    size_t off = 0;
+    for (int k = 0; k < 6; ++k)
+        if (host[k]) {
+            dev[k] = d_out + off;
+            off += per[k] * n;
+        }
+    const SceneOut o{dev[0], dev[1], dev[2], dev[3], dev[4], dev[5]};
     StepArgs a = args(kSceneActive);
-    launch_gather_scene(a, launch_, (uint32_t)sid, d_out, (uint32_t)n, track ? 1 : 0, stream_);
-    std::vector<float> h((size_t)nf * n);
-    HIP_TRY(hipMemcpyAsync(h.data(), d_out, h.size() * 4, hipMemcpyDeviceToHost, stream_));
+    launch_gather_scene(a, launch_, (uint32_t)sid, (uint32_t)n, o, stream_);
+    for (int k = 0; k < 6; ++k)
+        if (host[k]) HIP_TRY(hipMemcpyAsync(host[k], dev[k], per[k] * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_TRY(hipFreeAsync(d_out, stream_));
     HIP_TRY(hipStreamSynchronize(stream_));
-    for (size_t i = 0; i < n; ++i) {
-        size_t o = 0;
-        for (int e = 0; e < 3; ++e, o += n)
-            if (x) x[i * 3 + e] = h[o + i];
-        for (int e = 0; e < 3; ++e, o += n)
-            if (v) v[i * 3 + e] = h[o + i];
-        for (int e = 0; e < 9; ++e, o += n)
-            if (Fe) Fe[i * 9 + e] = h[o + i];
-        if (Jp) Jp[i] = h[o + i];
-        o += n;
-        if (V0) V0[i] = h[o + i];
-        o += n;
-        if (track)
-            for (int e = 0; e < 9; ++e, o += n)
-                if (Fp) Fp[i * 9 + e] = h[o + i];
-    }
     return MPM_OK;
 }
 

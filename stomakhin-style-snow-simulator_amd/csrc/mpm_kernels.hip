@@ -1553,24 +1553,56 @@ __global__ __launch_bounds__(256) void k_init_stress(StepArgs a, uint32_t first,
 }
 
 // Gathers scene `sid`'s particles into pid order (download staging).
-__global__ __launch_bounds__(256) void k_gather_scene(StepArgs a, uint32_t sid, float* out, uint32_t nscene,
-                                                      int track) {
-    const uint32_t n = a.cnt->n_particles;
-    // out layout: x3 v3 F9 Jp V0 [Fp9], each block of nscene floats
+// A new scene's particles into the FP32 buffers at first .. first + n - 1
+// (pid = index in the scene).  J_P = det F_P is formed in double exactly as
+// written (no contraction), F_P kept only with the full-F_P layout.
+__global__ __launch_bounds__(256) void k_put_scene(StepArgs a, uint32_t first, uint32_t n, uint32_t sid, SceneIn in) {
+#pragma clang fp contract(off)
+    const bool track = a.cur.Fp[0] != nullptr;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (a.cur.sid[i] != sid) continue;
-        const uint32_t p = a.cur.pid[i];
+        const uint32_t d = first + i;
+        for (int e = 0; e < 3; ++e) {
+            comp(a.cur.x, a.cur, e, d) = (float)in.x[(size_t)i * 3 + e];
+            comp(a.cur.v, a.cur, e, d) = (float)in.v[(size_t)i * 3 + e];
+        }
+        double P[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+        if (in.Fp)
+            for (int e = 0; e < 9; ++e) P[e] = in.Fp[(size_t)i * 9 + e];
+        for (int e = 0; e < 9; ++e) {
+            comp(a.cur.F, a.cur, e, d) = in.Fe ? (float)in.Fe[(size_t)i * 9 + e] : (e % 4 == 0 ? 1.0f : 0.0f);
+            if (track) comp(a.cur.Fp, a.cur, e, d) = (float)P[e];
+        }
+        const double jp = P[0] * (P[4] * P[8] - P[5] * P[7]) - P[1] * (P[3] * P[8] - P[5] * P[6]) +
+                          P[2] * (P[3] * P[7] - P[4] * P[6]);
+        at(a.cur.Jp, d) = (float)jp;
+        at(a.cur.m, d) = (float)in.m[i];
+        at(a.cur.V0, d) = in.V0 ? (float)in.V0[i] : 0.0f;
+        at(a.cur.E, d) = in.E ? (float)in.E[i] : 0.0f;
+        at(a.cur.pid, d) = i;
+        at(a.cur.sid, d) = sid;
+    }
+}
+
+// A scene's particles in their original order, in the reference's host
+// layout (doubles; x and v 3 per particle, F_E and F_P 9 row-major, J_P and
+// V_p0 1), written straight into the requested outputs (null: not wanted),
+// so the host copies exactly the bytes the caller asked for.
+__global__ __launch_bounds__(256) void k_gather_scene(StepArgs a, uint32_t sid, uint32_t nscene, SceneOut o) {
+    const uint32_t n = a.cnt->n_particles;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (at(a.cur.sid, i) != sid) continue;
+        const uint32_t p = at(a.cur.pid, i);
         if (p >= nscene) continue;
-        size_t o = 0;
-        for (int e = 0; e < 3; ++e, o += nscene) out[o + p] = a.cur.x[e][i];
-        for (int e = 0; e < 3; ++e, o += nscene) out[o + p] = a.cur.v[e][i];
-        for (int e = 0; e < 9; ++e, o += nscene) out[o + p] = a.cur.F[e][i];
-        out[o + p] = a.cur.Jp[i];
-        o += nscene;
-        out[o + p] = a.cur.V0[i];
-        o += nscene;
-        if (track)
-            for (int e = 0; e < 9; ++e, o += nscene) out[o + p] = a.cur.Fp[e][i];
+        if (o.x)
+            for (int e = 0; e < 3; ++e) o.x[(size_t)p * 3 + e] = comp(a.cur.x, a.cur, e, i);
+        if (o.v)
+            for (int e = 0; e < 3; ++e) o.v[(size_t)p * 3 + e] = comp(a.cur.v, a.cur, e, i);
+        if (o.Fe)
+            for (int e = 0; e < 9; ++e) o.Fe[(size_t)p * 9 + e] = comp(a.cur.F, a.cur, e, i);
+        if (o.Fp)
+            for (int e = 0; e < 9; ++e) o.Fp[(size_t)p * 9 + e] = comp(a.cur.Fp, a.cur, e, i);
+        if (o.Jp) o.Jp[p] = at(a.cur.Jp, i);
+        if (o.V0) o.V0[p] = at(a.cur.V0, i);
     }
 }
 
@@ -1674,9 +1706,15 @@ void launch_init_stress(const StepArgs& a, uint32_t first, uint32_t n, hipStream
 
 void launch_set_count(Counters* c, uint32_t n, hipStream_t s) { k_set_count<<<1, 64, 0, s>>>(c, n); }
 
-void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, float* out, uint32_t nscene, int track,
+void launch_put_scene(const StepArgs& a, const LaunchCfg& c, uint32_t first, uint32_t n, uint32_t sid,
+                      const SceneIn& in, hipStream_t s) {
+    const uint32_t g = (n + 255) / 256 < (uint32_t)c.grid_p ? (n + 255) / 256 : (uint32_t)c.grid_p;
+    k_put_scene<<<g, 256, 0, s>>>(a, first, n, sid, in);
+}
+
+void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, uint32_t nscene, const SceneOut& o,
                          hipStream_t s) {
-    k_gather_scene<<<c.grid_p, 256, 0, s>>>(a, sid, out, nscene, track);
+    k_gather_scene<<<c.grid_p, 256, 0, s>>>(a, sid, nscene, o);
 }
 
 void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s) {

@@ -43,7 +43,19 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
 void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
 void launch_init_stress(const StepArgs& a, uint32_t first, uint32_t n, hipStream_t s);
 void launch_set_count(Counters* c, uint32_t n, hipStream_t s);
-void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, float* out, uint32_t nscene, int track,
+// Device outputs of a scene download, in the reference's host layout
+// (doubles); a null member is not gathered.
+struct SceneOut {
+    double *x, *v, *Fe, *Fp, *Jp, *V0;
+};
+// Device copies of a scene's host arrays (reference layout, doubles); x, v
+// and m are required, a null V0, Fe, Fp or E takes its default (0, I, I, 0).
+struct SceneIn {
+    const double *x, *v, *m, *V0, *Fe, *Fp, *E;
+};
+void launch_put_scene(const StepArgs& a, const LaunchCfg& c, uint32_t first, uint32_t n, uint32_t sid,
+                      const SceneIn& in, hipStream_t s);
+void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, uint32_t nscene, const SceneOut& o,
                          hipStream_t s);
 void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s);
 

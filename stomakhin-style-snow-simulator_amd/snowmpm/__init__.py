@@ -355,17 +355,23 @@ class Engine:
         """Like step() but returns the status code instead of raising."""
         return lib.mpm_step(self.h, int(nsteps), None)
 
-    def download(self, sid):
+    def download(self, sid, fields=None):
+        """The scene's state in its original particle order: x, v (n x 3), Fe
+        (and Fp with track_fp) (n x 3 x 3), Jp, V0 (n); fields limits it to
+        the names given (only those are gathered and copied)."""
         n = self.count(sid)
-        x = np.zeros((n, 3))
-        v = np.zeros((n, 3))
-        Fe = np.zeros((n, 9))
-        Jp = np.zeros(n)
-        V0 = np.zeros(n)
-        Fp = np.zeros((n, 9)) if self.track_fp else None
+        want = set(fields) if fields is not None else {"x", "v", "Fe", "Jp", "V0", "Fp"}
+        x = np.zeros((n, 3)) if "x" in want else None
+        v = np.zeros((n, 3)) if "v" in want else None
+        Fe = np.zeros((n, 9)) if "Fe" in want else None
+        Jp = np.zeros(n) if "Jp" in want else None
+        V0 = np.zeros(n) if "V0" in want else None
+        Fp = np.zeros((n, 9)) if self.track_fp and "Fp" in want else None
         _check(lib.mpm_scene_download(self.h, int(sid), _dptr(x), _dptr(v), _dptr(Fe), _dptr(Fp), _dptr(Jp),
                                       _dptr(V0)))
-        out = dict(x=x, v=v, Fe=Fe.reshape(n, 3, 3), Jp=Jp, V0=V0)
+        out = {k: a for k, a in (("x", x), ("v", v), ("Jp", Jp), ("V0", V0)) if a is not None}
+        if Fe is not None:
+            out["Fe"] = Fe.reshape(n, 3, 3)
         if Fp is not None:
             out["Fp"] = Fp.reshape(n, 3, 3)
         return out

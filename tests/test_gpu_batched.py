@@ -105,3 +105,38 @@ def test_remove_and_add_between_steps_keeps_results():
     for k in ("x", "v", "Fe", "Jp"):
         assert np.array_equal(a0[k], g0[k]), k
         assert np.array_equal(a2[k], g2[k]), k
+
+
+def test_upload_download_round_trip():
+    """Scene upload and download convert on the GPU between the reference's
+    host layout (doubles, per-particle x/v/F rows) and the FP32 buffers: the
+    state comes back as the float32 rounding of what went in (J_P = det F_P
+    formed in double), in the input order, and a download of some fields
+    returns exactly those."""
+    rng = np.random.default_rng(4)
+    n = 1500
+    x = rng.uniform(-1, 1, (n, 3))
+    v = rng.uniform(-2, 2, (n, 3))
+    m = rng.uniform(1e-4, 1e-3, n)
+    V0 = rng.uniform(1e-7, 1e-6, n)
+    E = rng.uniform(1e5, 2e5, n)
+    Fe = (np.eye(3) + rng.normal(0, 0.05, (n, 3, 3))).reshape(n, 9)
+    Fp = (np.eye(3) + rng.normal(0, 0.05, (n, 3, 3))).reshape(n, 9)
+    prm = snowmpm.params(1e-4, 0.02, *snowmpm.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    f32 = lambda a: a.astype(np.float32).astype(np.float64)
+    with snowmpm.Engine(capacity=n + 64, track_fp=True) as eng:
+        eng.add_scene(prm, [], x[:7], v[:7], m[:7])  # another scene first: the second lands at an offset
+        sid = eng.add_scene(prm, [snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)], x, v, m, V0=V0, Fe=Fe, Fp=Fp,
+                            E=E)
+        full = eng.download(sid)
+        part = eng.download(sid, fields=("x", "Jp"))
+    assert np.array_equal(full["x"], f32(x)) and np.array_equal(full["v"], f32(v))
+    assert np.array_equal(full["Fe"].reshape(n, 9), f32(Fe)) and np.array_equal(full["Fp"].reshape(n, 9), f32(Fp))
+    assert np.array_equal(full["V0"], f32(V0))
+    P = Fp.reshape(n, 3, 3)
+    det = (P[:, 0, 0] * (P[:, 1, 1] * P[:, 2, 2] - P[:, 1, 2] * P[:, 2, 1])
+           - P[:, 0, 1] * (P[:, 1, 0] * P[:, 2, 2] - P[:, 1, 2] * P[:, 2, 0])
+           + P[:, 0, 2] * (P[:, 1, 0] * P[:, 2, 1] - P[:, 1, 1] * P[:, 2, 0]))
+    assert np.array_equal(full["Jp"], f32(det))
+    assert set(part) == {"x", "Jp"}
+    assert np.array_equal(part["x"], full["x"]) and np.array_equal(part["Jp"], full["Jp"])
