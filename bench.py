@@ -229,8 +229,9 @@ def main():
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU path "
-              "(sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: the engine runs on a single MI355X and has no multi-GPU path; "
+              "scenes are independent, so more GPUs would each run their own share of the scenes",
+              file=sys.stderr)
         sys.exit(2)
 
     import snowmpm
