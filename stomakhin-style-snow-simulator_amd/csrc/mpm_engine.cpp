@@ -130,7 +130,7 @@ int Engine::init(const mpm_config& cfg) {
     if ((rc = dalloc((void**)&fb_.L, N * 36))) return rc;
 
     grid_.mask = H_ - 1;
-    grid_.max_probe = 4096;
+    grid_.max_probe = std::min<uint32_t>(4096u, H_);  // a full table is scanned once at most
     if ((rc = dalloc((void**)&grid_.keys, (size_t)H_ * 8))) return rc;
     if ((rc = dalloc((void**)&grid_.cellcnt, (size_t)H_ * 64 * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.celloff, (size_t)H_ * 64 * 4))) return rc;

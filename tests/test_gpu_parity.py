@@ -54,13 +54,13 @@ def test_freefall_closed_form():
     assert np.abs(got["Fe"] - np.eye(3)).max() < 1e-5
 
 
-@pytest.mark.parametrize("kind,scale,index", [(0, 1.0, 0), (1, 0.15, 0), (2, 0.05, 0), (3, 0.1, 0), (3, 0.1, 1),
-                                               (3, 0.1, 2)])
+@pytest.mark.parametrize("kind,scale,index", [(0, 1.0, 0), (1, 0.15, 0), (2, 0.05, 0), (3, 1.0, 0), (3, 1.0, 1),
+                                               (3, 1.0, 2), (3, 0.1, 0)])
 def test_ten_steps_match_oracle(kind, scale, index):
-    """The 64,000-particle cube at full size, and the other workloads at reduced
-    size: the two tori, the slab (per-particle E, box collider) and scenes of
-    the batched distribution (box, sphere and torus shapes, random velocities
-    up to 8 m/s)."""
+    """The 64,000-particle cube and three scenes of the batched distribution
+    (46,930, 84,643 and 3,561 particles; box, sphere and torus shapes, random
+    velocities up to 8 m/s) at full size, and the two tori, the slab
+    (per-particle E, box collider) and a batched scene at reduced size."""
     sc = _scene(kind, scale, index=index)
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
@@ -291,6 +291,50 @@ def test_nan_is_reported():
         st, _ = eng.status(sid)
     assert rc == snowmpm.ERR_DEVICE
     assert st & snowmpm.STATUS_NAN
+
+
+def _block_lattice(n, dx=0.02):
+    """One particle per 4^3-node block of an n x n x n arrangement, 1.04 m (13
+    blocks) up: its stencil base sits at node 1 of its block, so the scene has
+    n^3 home blocks and (n + 1)^3 blocks in all."""
+    i = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (4 * i + 2.5) * dx + np.array([0.0, 1.04, 0.0])
+    prm = snowmpm.params(1e-4, dx, *snowmpm.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    return dict(x=x, v=np.zeros_like(x), m=np.full(len(x), 4e-4), params=prm,
+                colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
+
+
+def test_hash_size_does_not_change_bits():
+    """Block hash slots are assigned by insertion races, but nothing summed
+    depends on them: a 128-slot table (the cube's 64 blocks fill half of it,
+    long probe chains) gives the same bits as the default 65,536 slots."""
+    sc = _scene(0, 0.5)
+    out = []
+    for hl in (7, 0):
+        with snowmpm.Engine(capacity=len(sc["x"]) + 64, hash_log2=hl) as eng:
+            sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+            eng.step(10)
+            out.append(eng.download(sid))
+            assert eng.status(sid)[0] == 0
+    for k in ("x", "v", "Fe", "Jp"):
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
+@pytest.mark.parametrize("hash_log2,expect,absent", [(9, snowmpm.STATUS_OVERFLOW, snowmpm.STATUS_HASH_FULL),
+                                                     (8, snowmpm.STATUS_HASH_FULL, 0)])
+def test_small_hash_is_reported(hash_log2, expect, absent):
+    """343 home blocks, 512 blocks in all.  With 512 slots every block is
+    inserted but the P2G partial sums hold only 256 home blocks (overflow);
+    with 256 slots the table fills.  Either way the step fails with
+    ERR_DEVICE and the scene's status names the cause."""
+    sc = _block_lattice(7)
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64, hash_log2=hash_log2) as eng:
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        rc = eng.step_rc(1)
+        st, _ = eng.status(sid)
+    assert rc == snowmpm.ERR_DEVICE
+    assert st & expect
+    assert not st & absent
 
 
 def test_fp64_fallback_matches_oracle():
