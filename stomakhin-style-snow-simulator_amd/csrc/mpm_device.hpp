@@ -57,6 +57,7 @@ struct SceneDev {
     int nproject;                            // colliders with the projection flag
     ColliderDev coll[kMaxColliders];
 };
+static_assert(sizeof(SceneDev) % 4 == 0, "scene records are copied as 32-bit words");
 constexpr int kSceneActive = 1;
 constexpr int kSceneSetup = 2;
 

@@ -1223,6 +1223,12 @@ template <bool TRACK, bool PERMUTE, bool PROJECT>
 __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
     __shared__ float4 va[512];  // [Z][Y][X]: v0 v1 v2 u0
     __shared__ float2 ub[512];  // [Z][Y][X]: u1 u2
+    // The block's scene (parameters and colliders), copied to LDS once per
+    // block.  Read from global memory after the particle stores, every
+    // scene or collider access would be a vector load whose wait
+    // (vmcnt(0)) also waits for those stores to be acknowledged; G2P spent
+    // ~90 us of 0.45 ms on the 8M slab in such waits in the collider loop.
+    __shared__ SceneDev ssc;
     const uint32_t nh = a.cnt->n_home;
     // Threads 32 tg .. 32 tg + 31 stage neighbour blocks tg, tg + T/32, ...
     // (T threads per workgroup).
@@ -1243,7 +1249,12 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
         uint32_t sid;
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
-        const SceneDev& sc = a.scenes[sid];
+        {  // after the barrier above: no thread still reads the previous block's copy
+            const int* src = reinterpret_cast<const int*>(&a.scenes[sid]);
+            int* dst = reinterpret_cast<int*>(&ssc);
+            for (int e = (int)threadIdx.x; e < (int)(sizeof(SceneDev) / 4); e += kG2PThreads) dst[e] = src[e];
+        }
+        const SceneDev& sc = ssc;
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
