@@ -68,8 +68,8 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
 }
 
 // Base node of the particle's stencil, floor(x/dx) - 1 (grid.cpp:64-67).
-__device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1, float x2, int b[3], float f[3]) {
-    const float g0 = x0 * sc.inv_dx, g1 = x1 * sc.inv_dx, g2 = x2 * sc.inv_dx;
+__device__ __forceinline__ void base_node(float inv_dx, float x0, float x1, float x2, int b[3], float f[3]) {
+    const float g0 = x0 * inv_dx, g1 = x1 * inv_dx, g2 = x2 * inv_dx;
     const float c0 = floorf(g0), c1 = floorf(g1), c2 = floorf(g2);
     f[0] = g0 - c0;
     f[1] = g1 - c1;
@@ -78,11 +78,11 @@ __device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1
     b[1] = (int)c1 - 1;
     b[2] = (int)c2 - 1;
 }
+__device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1, float x2, int b[3], float f[3]) {
+    base_node(sc.inv_dx, x0, x1, x2, b, f);
+}
 
 // ---------------------------------------------------------------------------
-
-
-
 
 __global__ void k_set_count(Counters* c, uint32_t n) {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->n_particles = n;
@@ -102,19 +102,36 @@ struct BinItem {
     uint32_t l, sid;
     bool want;
 };
-__device__ __forceinline__ BinItem bin_item(const StepArgs& a, uint32_t i, uint32_t n, uint32_t sid, float x0, float x1,
-                                            float x2) {
+// The scene fields binning needs, cached per wave for the scene of the
+// wave's first particle (nearly every particle of a wave, the buffer being
+// sorted by block).  Loaded from global memory after the previous pass's
+// stores, they would make every pass wait for those stores' acknowledgement.
+struct SceneCache {
+    uint32_t sid;
+    int flags;
+    float inv_dx;
+};
+__device__ __forceinline__ BinItem bin_item(const StepArgs& a, const SceneCache& sc_c, uint32_t i, uint32_t n,
+                                            uint32_t sid, float x0, float x1, float x2) {
     BinItem it{kEmptyKey, 0u, sid, false};
     if (i >= n) return it;
-    const SceneDev& sc = a.scenes[sid];
-    if (!(sc.flags & a.pass_flag)) return it;
+    int flags;
+    float inv_dx;
+    if (sid == sc_c.sid) {
+        flags = sc_c.flags;
+        inv_dx = sc_c.inv_dx;
+    } else {
+        flags = a.scenes[sid].flags;
+        inv_dx = a.scenes[sid].inv_dx;
+    }
+    if (!(flags & a.pass_flag)) return it;
     if (!finite3(x0, x1, x2)) {
         flag(a.st, sid, kStatusNaN);
         return it;
     }
     int b[3];
     float f[3];
-    base_node(sc, x0, x1, x2, b, f);
+    base_node(inv_dx, x0, x1, x2, b, f);
     const int bx = b[0] >> 2, by = b[1] >> 2, bz = b[2] >> 2;
     const int lim = kCoordBias - 2;
     if (bx < -lim || bx > lim || by < -lim || by > lim || bz < -lim || bz > lim) {
@@ -201,10 +218,15 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
         }
     };
     load(wbase);
+    SceneCache scc;
+    scc.sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)nsid[0]);
+    scc.flags = a.scenes[scc.sid].flags;
+    scc.inv_dx = a.scenes[scc.sid].inv_dx;
     for (; wbase < n; wbase += stride) {
         BinItem it[2];
 #pragma unroll
-        for (int s = 0; s < 2; ++s) it[s] = bin_item(a, wbase + 64 * s + lane, n, nsid[s], nx[s][0], nx[s][1], nx[s][2]);
+        for (int s = 0; s < 2; ++s)
+            it[s] = bin_item(a, scc, wbase + 64 * s + lane, n, nsid[s], nx[s][0], nx[s][1], nx[s][2]);
         load(wbase + stride);
         // block keys: one insert per distinct key and set
         const int kl0 = bin_leader64<false>(it[0].key, it[0].want);
