@@ -1259,6 +1259,16 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
     const int tg = (int)(threadIdx.x >> 5), l32 = (int)(threadIdx.x & 31);
     for (uint32_t blk = blockIdx.x; blk < nh; blk += G) {
         const uint32_t slot = a.g.home_list[blk];
+        uint32_t sid;
+        int bx, by, bz;
+        unpack_key(a.g.keys[slot], sid, bx, by, bz);
+        // The scene record is loaded into registers with the staging loads and
+        // written to LDS after the barrier (at most two words per thread).
+        constexpr int kSceneWords = (int)(sizeof(SceneDev) / 4);
+        static_assert(kSceneWords <= 2 * kG2PThreads, "two scene words per thread");
+        const int* ssrc = reinterpret_cast<const int*>(&a.scenes[sid]);
+        const int t0 = (int)threadIdx.x, t1 = (int)threadIdx.x + kG2PThreads;
+        const int sw0 = t0 < kSceneWords ? ssrc[t0] : 0, sw1 = t1 < kSceneWords ? ssrc[t1] : 0;
         {
             float stg[8 / kGroups][kG2PStage];
 #pragma unroll
@@ -1268,21 +1278,23 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
 #pragma unroll
             for (int k = 0; k < 8 / kGroups; ++k) g2p_stage_store(va, ub, tg + k * kGroups, l32, stg[k]);
         }
-        uint32_t sid;
-        int bx, by, bz;
-        unpack_key(a.g.keys[slot], sid, bx, by, bz);
         {  // after the barrier above: no thread still reads the previous block's copy
-            const int* src = reinterpret_cast<const int*>(&a.scenes[sid]);
             int* dst = reinterpret_cast<int*>(&ssc);
-            for (int e = (int)threadIdx.x; e < (int)(sizeof(SceneDev) / 4); e += kG2PThreads) dst[e] = src[e];
+            if (t0 < kSceneWords) dst[t0] = sw0;
+            if (t1 < kSceneWords) dst[t1] = sw1;
         }
         const SceneDev& sc = ssc;
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
         __syncthreads();
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
         const float inv_dx = sc.inv_dx, dt = sc.dt, alpha = sc.alpha;
-        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += kG2PThreads) {
-            const uint32_t i = at(a.bin.sidx, j);
+        // The next particle's sorted index is loaded before this particle's
+        // stores are issued, so waiting for it does not wait for them.
+        const uint32_t jend = off + cnt;
+        uint32_t i_n = off + threadIdx.x < jend ? at(a.bin.sidx, off + threadIdx.x) : 0u;
+        for (uint32_t j = off + threadIdx.x; j < jend; j += kG2PThreads) {
+            const uint32_t i = i_n;
+            if (j + kG2PThreads < jend) i_n = at(a.bin.sidx, j + kG2PThreads);
             const float x0 = comp(a.cur.x, a.cur, 0, i), x1 = comp(a.cur.x, a.cur, 1, i), x2 = comp(a.cur.x, a.cur, 2, i);
             float v0 = comp(a.cur.v, a.cur, 0, i), v1 = comp(a.cur.v, a.cur, 1, i), v2 = comp(a.cur.v, a.cur, 2, i);
             int b[3];
