@@ -29,7 +29,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "stomakhin-style-snow-simulator_amd"))
 
 HBM_PEAK_GBS = 8000.0
+MFMA_FP32_PEAK_TFLOPS = 157.3
 ALG_BYTES = {"p2g": 84.1, "g2p": 131.5, "grid": 11.5}
+# Useful multiply-adds per particle in P2G: per stencil node, mass w m (1),
+# momentum w m v (3) and force M grad w (3 x 3): 64 x 13 = 832.
+P2G_MACS_PER_PARTICLE = 832
 
 
 def cpu_baseline(seed):
@@ -274,6 +278,14 @@ def main():
     if os.path.exists(tf):
         with open(tf) as f:
             traffic = json.load(f).get(dom)
+    # both particle kernels against HBM; P2G also against the FP32 matrix peak
+    kernel_roofline = {}
+    for k in ("p2g", "g2p"):
+        gbs = ALG_BYTES[k] * n / (per_kernel[k] * 1e-3) / 1e9
+        kernel_roofline[k] = {"ms": round(per_kernel[k], 4), "hbm_gbs": round(gbs, 1),
+                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    tfl = 2.0 * P2G_MACS_PER_PARTICLE * n / (per_kernel["p2g"] * 1e-3) / 1e12
+    kernel_roofline["p2g"].update({"mfma_tflops": round(tfl, 2), "mfma_frac": round(tfl / MFMA_FP32_PEAK_TFLOPS, 4)})
     out = {
         "metric": "particle-steps/s (8M-particle slab)",
         "value": value,
@@ -294,6 +306,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
         "kernel_ms": {k: round(v, 4) for k, v in kt.items() if k != "steps"},
+        "kernel_roofline": kernel_roofline,
         "status": st,
         "fp64_fallbacks": fb,
         # Host buffers crossing PCIe (not part of value): scene upload from double
