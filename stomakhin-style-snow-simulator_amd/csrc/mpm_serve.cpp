@@ -62,9 +62,16 @@ int Engine::autotune(int spc, int chosen[4]) {
     if (spc < 1) spc = 1;
     int rc = setup();
     if (rc) return rc;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    struct Events {  // destroyed on every return path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    HIP_TRY(hipEventCreate(&ev.e0));
+    HIP_TRY(hipEventCreate(&ev.e1));
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     auto run = [&](const LaunchCfg& c, double* ms) -> int {
         launch_ = c;
         HIP_TRY(hipEventRecord(e0, stream_));
@@ -107,8 +114,6 @@ int Engine::autotune(int spc, int chosen[4]) {
     }
     launch_ = best;
     drop_graphs();
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (chosen) {
         chosen[0] = best.grid_p;
         chosen[1] = best.grid_b;
@@ -129,6 +134,12 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
         set_error("mpm_serve: bad arguments");
         return MPM_ERR_ARG;
     }
+    for (int k = 0; k < nreq; ++k)
+        if (R[k].steps < 0 || R[k].ncolliders < 0 ||
+            R[k].ncolliders > (int)(sizeof R[k].colliders / sizeof R[k].colliders[0])) {
+            set_error("mpm_serve: a request has steps < 0 or more colliders than mpm_request holds");
+            return MPM_ERR_ARG;
+        }
     if (max_chunk <= 0) max_chunk = 25;
     std::memset(st, 0, sizeof *st);
     using clk = std::chrono::steady_clock;

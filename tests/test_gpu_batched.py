@@ -63,6 +63,17 @@ def test_served_results_equal_alone():
     assert res["p99_s"] >= res["p50_s"] > 0
 
 
+def test_serve_rejects_a_negative_step_count():
+    sc = _scenes(1)[0]
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as eng:
+        with pytest.raises(snowmpm.MpmError) as e:
+            eng.serve([dict(sc, arrival_s=0.0, steps=-1)])
+        assert e.value.code == snowmpm.ERR_ARG
+        # the engine stays usable
+        res = eng.serve([dict(sc, arrival_s=0.0, steps=2)])
+    assert res["completed"] == 1
+
+
 def test_plain_serving_mode():
     scs = _scenes(3)
     reqs = [dict(sc, arrival_s=0.0, steps=5) for sc in scs]
