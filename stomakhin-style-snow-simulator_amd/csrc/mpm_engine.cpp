@@ -204,6 +204,21 @@ int Engine::upload_scene_table(int sid) {
     return MPM_OK;
 }
 
+namespace {
+// Stream-ordered staging memory, freed on every return path.
+struct StagingBuffer {
+    double* p = nullptr;
+    hipStream_t s;
+    explicit StagingBuffer(hipStream_t st) : s(st) {}
+    hipError_t release() {
+        const hipError_t e = p ? hipFreeAsync(p, s) : hipSuccess;
+        p = nullptr;
+        return e;
+    }
+    ~StagingBuffer() { (void)release(); }
+};
+}  // namespace
+
 int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll, size_t n, const double* x,
                       const double* v, const double* m, const double* V0, const double* Fe, const double* Fp,
                       const double* E, int* out_sid) {
@@ -244,8 +259,9 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     size_t total = 0;
     for (int k = 0; k < 7; ++k)
         if (host[k]) total += per[k] * n;
-    double* d_in = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&d_in, total * sizeof(double), stream_));
+    StagingBuffer stage(stream_);
+    HIP_TRY(hipMallocAsync((void**)&stage.p, total * sizeof(double), stream_));
+    double* const d_in = stage.p;
     const double* dev[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t off = 0;
     for (int k = 0; k < 7; ++k)
@@ -260,7 +276,7 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
                          stream_);
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipFreeAsync(d_in, stream_));
+    HIP_TRY(stage.release());
     SceneDev& sd = h_scenes_[sid];
     std::memset(&sd, 0, sizeof sd);
     sd.dx = (float)prm.dx;
@@ -444,8 +460,11 @@ int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp
     for (int k = 0; k < 6; ++k)
         if (host[k]) total += per[k] * n;
     if (total == 0 || n == 0) return MPM_OK;
-    double* d_out = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&d_out, total * sizeof(double), stream_));
+    StagingBuffer stage(stream_);
+    HIP_TRY(hipMallocAsync((void**)&stage.p, total * sizeof(double), stream_));
+    double* const d_out = stage.p;
+    // zeros for particles the scene no longer holds (a failed pass can drop them)
+    HIP_TRY(hipMemsetAsync(d_out, 0, total * sizeof(double), stream_));
     double* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t off = 0;
     for (int k = 0; k < 6; ++k)
@@ -458,7 +477,7 @@ int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp
     launch_gather_scene(a, launch_, (uint32_t)sid, (uint32_t)n, o, stream_);
     for (int k = 0; k < 6; ++k)
         if (host[k]) HIP_TRY(hipMemcpyAsync(host[k], dev[k], per[k] * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
-    HIP_TRY(hipFreeAsync(d_out, stream_));
+    HIP_TRY(stage.release());
     HIP_TRY(hipStreamSynchronize(stream_));
     return MPM_OK;
 }
