@@ -144,6 +144,28 @@ def test_contact_plasticity_matches_oracle():
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
+def test_eight_colliders_last_one_hit():
+    """The contact case with the ground plane as the last of eight colliders
+    (seven far-away spheres and boxes first): every collider of the scene
+    record reaches the particle and grid collision loops, in list order."""
+    sc = _scene(0, 0.3)
+    sc["v"] = np.tile(np.array([1.0, -6.0, 0.5]), (len(sc["x"]), 1))
+    sc["x"] = sc["x"] - np.array([0, 0.015, 0])
+    far = [snowmpm.make_sphere((5.0 + k, 3.0, -4.0), 0.2 + 0.1 * k, 0.1 * k) for k in range(4)]
+    far += [snowmpm.make_box((-6.0, 2.0 + k, 7.0), (0.3, 0.2, 0.1 * (k + 1)), 0.3) for k in range(3)]
+    sc["colliders"] = far + list(sc["colliders"])
+    assert len(sc["colliders"]) == 8
+    nsteps = 30
+    ref = mpmref.run_oracle(sc, nsteps)
+    got = mpmref.run_gpu(sc, nsteps)
+    jp_ref = np.linalg.det(ref["Fp"])
+    assert (jp_ref < 0.999).any(), "scenario must plastify"
+    ex, ev = mpmref.rel_errors(got, ref, sc)
+    print(f"eight colliders: rel err x {ex:.3e} v {ev:.3e}")
+    assert ex < TOL and ev < 1e-3
+    assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
+
+
 def test_sparse_blocks_exceed_the_grid():
     """21,952 particles 0.1 m apart, each in its own home block: more home
     blocks than the P2G and G2P grids hold workgroups (the persistent-loop
