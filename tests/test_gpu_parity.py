@@ -359,6 +359,32 @@ def test_small_hash_is_reported(hash_log2, expect, absent):
     assert not st & absent
 
 
+def test_far_particle_is_reported():
+    """A particle beyond the block-coordinate range (|x| > ~5 km at dx 0.02)
+    is left out of the grid and the step reports MPM_STATUS_RANGE."""
+    sc = _scene(0, 0.2)
+    sc["x"] = sc["x"].copy()
+    sc["x"][7] = (1.0e4, 1.0, 0.0)
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as eng:
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        rc = eng.step_rc(1)
+        st, _ = eng.status(sid)
+    assert rc == snowmpm.ERR_DEVICE
+    assert st & snowmpm.STATUS_RANGE
+
+
+def test_capacity_is_enforced():
+    sc = _scene(0, 0.2)
+    with snowmpm.Engine(capacity=len(sc["x"]) - 1) as eng:
+        with pytest.raises(snowmpm.MpmError) as e:
+            eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        assert e.value.code == snowmpm.ERR_CAPACITY
+        half = len(sc["x"]) // 2
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"][:half], sc["v"][:half], sc["m"][:half])
+        eng.step(2)
+        assert eng.status(sid)[0] == 0
+
+
 def test_fp64_fallback_matches_oracle():
     """cond(trial F_E) = 1.3/0.8 > 1.5 sends every particle's first plasticity
     update (and the upload's stress) through the FP64 SVD; the clamp then
