@@ -13,6 +13,8 @@ Statistics (FP64 sums over the downloaded state):
 Measured on MI355X: 2e-8, 2e-6, 5e-4, 1e-6 and per-particle x 2e-6, v 4e-5;
 the bounds below leave 10-100x margin for trajectory divergence.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -54,4 +56,35 @@ def test_long_run_aggregates_match_oracle():
     assert e_hist < 0.01
     assert e_comp < 1e-4
     assert ex < 1e-4 and ev < 1e-3
+    assert got["status"][0] == 0
+
+
+@pytest.mark.skipif(not os.environ.get("SNOWMPM_FULL_PARITY"),
+                    reason="the oracle needs one to two minutes for 2000 steps of 36,708 particles; "
+                           "set SNOWMPM_FULL_PARITY=1 to run it")
+def test_tori_collision_aggregates_match_oracle():
+    """The two-tori workload at 0.3 scale (36,708 particles, +-5 m/s, 3 cm
+    apart): 2000 steps (0.1 s) through their head-on collision, the step
+    count of the full-size workload.  Trajectories diverge after the impact,
+    so only the aggregate statistics are compared.  Measured on MI355X: centre
+    of mass 1.9e-6, momentum 6.8e-6, J_P histogram 5.5e-4, compacted volume
+    1.3e-4, 17,049 against 17,051 particles yielded; the bounds leave ~10x."""
+    sc = snowmpm.generate_scene(1, seed=11, scale=0.3)
+    steps = 2000
+    got = mpmref.run_gpu(sc, steps)
+    ref = mpmref.run_oracle(sc, steps)
+    sg = _stats(sc, got, got["Jp"].astype(np.float64))
+    sr = _stats(sc, ref, np.linalg.det(ref["Fp"]))
+    L = float(np.ptp(sc["x"][:, 0]))
+    e_com = np.abs(sg["com"] - sr["com"]).max() / L
+    e_P = np.abs(sg["P"] - sr["P"]).max() / (sr["M"] * 5.0)
+    e_hist = np.abs(sg["hist"] - sr["hist"]).sum() / 2
+    e_comp = abs(sg["comp"] - sr["comp"]) / sr["comp"]
+    print(f"tori 2000 steps: com {e_com:.2e} momentum {e_P:.2e} J_P hist {e_hist:.2e} "
+          f"compacted {e_comp:.2e} (yielded {sg['yielded']} / {sr['yielded']})")
+    assert sr["yielded"] > len(sc["x"]) // 4          # the collision does compact the snow
+    assert e_com < 2e-5
+    assert e_P < 1e-4
+    assert e_hist < 0.01
+    assert e_comp < 2e-3
     assert got["status"][0] == 0
