@@ -98,7 +98,8 @@ typedef struct {
     int device;               /* HIP device ordinal */
     int permute_period;       /* steps between full re-sorts of the particle buffers; 0 = 16.
                                  Results do not depend on it (steps in between update the
-                                 particles at their index); 1 re-sorts every step */
+                                 particles at their index); 1 re-sorts every step.  Fast-moving
+                                 scenes run ~3% faster at 4, scenes at rest ~3% slower */
     double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
 } mpm_config;
 
