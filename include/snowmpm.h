@@ -148,11 +148,16 @@ int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
  * Status is checked once at the end. */
 int mpm_step_graph(mpm_engine* e, int nsteps);
 
-/* Chooses the persistent grid sizes of the step kernels by timing real steps
- * (results do not depend on the launch shape, so tuning steps are ordinary
- * steps).  steps_per_config >= 1; chosen (4 ints, may be NULL) receives
+/* Chooses the persistent grid sizes of the step kernels, then the re-sort
+ * period among 16, 8, 4 and 2 (16 steps per candidate), by timing real steps
+ * (results depend on neither, so tuning steps are ordinary steps).
+ * steps_per_config >= 1; chosen (4 ints, may be NULL) receives
  * {particle grid, wave-per-block grid, workgroup-per-block grid, k_blocks grid}. */
 int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]);
+
+/* The engine's current re-sort period (mpm_config.permute_period, or the one
+ * mpm_autotune chose).  No counterpart in the reference. */
+int mpm_permute_period(mpm_engine* e, int* period);
 
 /* ---- serving: continuous batching of scene requests ----
  * Each request brings its particles (host arrays, V0 computed by setup) and a

@@ -53,6 +53,7 @@ class Engine {
     int step(int nsteps, mpm_timings* t);
     int step_graph(int nsteps);
     int autotune(int steps_per_config, int chosen[4]);
+    int permute_period() const { return cfg_.permute_period; }
     int serve(mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* st);
     int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
     int scene_status(int sid, uint32_t* status, uint64_t* fb);

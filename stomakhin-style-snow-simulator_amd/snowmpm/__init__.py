@@ -102,6 +102,7 @@ _SIGS = {
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
+    "mpm_permute_period": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mpm_serve": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int, C.c_int, C.POINTER(ServeStats)]),
     "mpm_make_plane": (Collider, [_pd, _pd, C.c_double]),
     "mpm_make_box": (Collider, [_pd, _pd, C.c_double]),
@@ -310,7 +311,14 @@ class Engine:
     def autotune(self, steps_per_config=2):
         chosen = (C.c_int * 4)()
         _check(lib.mpm_autotune(self.h, int(steps_per_config), chosen))
-        return dict(grid_p=chosen[0], grid_b=chosen[1], grid_w=chosen[2], grid_k=chosen[3])
+        return dict(grid_p=chosen[0], grid_b=chosen[1], grid_w=chosen[2], grid_k=chosen[3],
+                    permute_period=self.permute_period())
+
+    def permute_period(self):
+        """Steps between full re-sorts of the particle buffers."""
+        p = C.c_int()
+        _check(lib.mpm_permute_period(self.h, C.byref(p)))
+        return p.value
 
     def serve(self, requests, mode=0, max_chunk=25, keep_outputs=False):
         """Runs the request loop.  requests: dicts with arrival_s, steps and a

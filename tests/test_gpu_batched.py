@@ -37,13 +37,17 @@ def test_autotune_keeps_results():
         assert all(v > 0 for v in chosen.values())
         # the tuner ran ordinary steps: count them from a reference run
         got = eng.download(sid)
+    assert chosen["permute_period"] in (2, 4, 8, 16)
     nsteps_tuner = None
-    for n in range(2, 20):
-        ref = mpmref.run_gpu(sc, n)
-        if np.array_equal(ref["x"], got["x"]):
-            nsteps_tuner = n
-            break
-    assert nsteps_tuner is not None
+    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as ref_eng:
+        rid = ref_eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        for n in range(1, 200):
+            ref_eng.step(1)
+            if np.array_equal(ref_eng.download(rid, fields=("x",))["x"], got["x"]):
+                nsteps_tuner = n
+                break
+    # launch-shape candidates (one step each) and 4 re-sort periods x 16 steps
+    assert nsteps_tuner is not None and nsteps_tuner > 64
 
 
 def test_served_results_equal_alone():
