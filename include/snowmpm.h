@@ -40,6 +40,7 @@ extern "C" {
 #define MPM_STATUS_OVERFLOW 0x2u  /* more occupied blocks than the P2G partial-sum buffer holds */
 #define MPM_STATUS_HASH_FULL 0x4u /* block hash table full */
 #define MPM_STATUS_RANGE 0x8u     /* particle outside the block-coordinate range */
+#define MPM_STATUS_REJECTED 0x100u /* serving: request larger than the engine's particle pool (host-side) */
 
 typedef struct mpm_engine mpm_engine;
 
@@ -159,38 +160,60 @@ int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]);
  * mpm_autotune chose).  No counterpart in the reference. */
 int mpm_permute_period(mpm_engine* e, int* period);
 
+/* Steps scene `scene` has taken since it was added (setup not counted).
+ * No counterpart in the reference, whose step() takes no scene handle. */
+int mpm_scene_steps(mpm_engine* e, int scene, uint64_t* steps);
+
 /* ---- serving: continuous batching of scene requests ----
  * Each request brings its particles (host arrays, V0 computed by setup) and a
  * step count; it is admitted once it has arrived (arrival_s after the call
- * starts) and its footprint (particles, estimated blocks) fits, steps with all
- * resident scenes in graph-replayed chunks, and is downloaded and retired as
- * soon as its steps are done.  mode 0: continuous batching; mode 1: the plain
- * per-scene GPU step, one request at a time in arrival order. */
+ * starts) and its device footprint fits (mpm_request_footprint: particle
+ * slots, grid blocks and P2G partial sums, in bytes of the engine's
+ * pools), steps with all resident scenes in graph-replayed chunks, and is
+ * downloaded and retired as soon as its steps are done.  A request whose
+ * scene raises a device status bit is retired at the end of that chunk with
+ * the bits in `status` (its outputs are not written); the others go on.
+ * mode 0: continuous batching; mode 1: the plain per-scene GPU step, one
+ * request at a time in arrival order. */
+#define MPM_MAX_COLLIDERS 8
 typedef struct {
     double arrival_s;
     int steps;
-    int ncolliders;
+    int ncolliders;   /* <= MPM_MAX_COLLIDERS */
     size_t n;
     const double* x;
     const double* v;
     const double* m;
     const double* E; /* may be NULL */
     mpm_params prm;
-    mpm_collider colliders[4];
+    mpm_collider colliders[MPM_MAX_COLLIDERS];
     double* out_x; /* n*3, may be NULL */
     double* out_v; /* n*3, may be NULL */
     double t_admit_s; /* filled in */
     double t_done_s;  /* filled in */
+    uint32_t status;  /* filled in: 0, or the device status bits that retired the request */
+    size_t footprint_bytes; /* filled in: device bytes reserved for the request */
 } mpm_request;
 
 typedef struct {
     double wall_s;
     double scenes_per_s;
-    double p50_s, p99_s, mean_s, max_s; /* completion latency = t_done - arrival */
+    double p50_s, p99_s, mean_s, max_s; /* completion latency = t_done - arrival, completed requests */
     double particle_steps;
     int completed;
     int max_resident;
+    int failed;              /* requests retired with a device status */
+    double max_resident_bytes; /* peak sum of the resident requests' footprints */
 } mpm_serve_stats;
+
+/* Device bytes request q would hold in engine e: n particle slots (both
+ * particle buffers, sort and fallback arrays), its grid blocks (hash slot,
+ * cell tables, neighbour and velocity arrays) and its home blocks' P2G
+ * partial sums.  Blocks are estimated from the bounding box of q's positions
+ * in 4^3-node blocks, with a 25% margin for the body spreading as it moves.
+ * pools (may be NULL) receives the engine's three pool sizes in bytes:
+ * particles, blocks (at the hash table's 50% load limit), partial sums. */
+int mpm_request_footprint(mpm_engine* e, const mpm_request* q, size_t* bytes, size_t pools[3]);
 
 int mpm_serve(mpm_engine* e, mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* stats);
 

@@ -145,16 +145,22 @@ def make_particle(x, v, m, V0, Fe=None, Fp=None, mu0=-1.0, lambda0=-1.0):
     return p
 
 
-def make_plane(point, normal, friction):
-    return lib.orc_make_plane(vec(*point), vec(*normal), float(friction))
+def _moving(c, velocity):
+    if velocity is not None:
+        c.velocity[:] = list(map(float, velocity))
+    return c
 
 
-def make_box(centre, half_extent, friction):
-    return lib.orc_make_box(vec(*centre), vec(*half_extent), float(friction))
+def make_plane(point, normal, friction, velocity=None):
+    return _moving(lib.orc_make_plane(vec(*point), vec(*normal), float(friction)), velocity)
 
 
-def make_sphere(centre, radius, friction):
-    return lib.orc_make_sphere(vec(*centre), float(radius), float(friction))
+def make_box(centre, half_extent, friction, velocity=None):
+    return _moving(lib.orc_make_box(vec(*centre), vec(*half_extent), float(friction)), velocity)
+
+
+def make_sphere(centre, radius, friction, velocity=None):
+    return _moving(lib.orc_make_sphere(vec(*centre), float(radius), float(friction)), velocity)
 
 
 def position_projection(p, cs):

@@ -20,6 +20,7 @@
 // std::runtime_error, as the reference's I/O does.
 #pragma once
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -84,12 +85,18 @@ struct Particle {
 
 // CollisionBody, collision.hpp:6-12: the device needs a POD description, so
 // the body carries one; phi/normal/velocity stay available on the host.
+// Only bodies built by make_plane (or make_box / make_sphere below) have a
+// description: step() throws for a body assembled from closures of its own
+// (arbitrary SDF closures cannot run on the GPU), and for a body whose
+// friction or velocity closure was changed afterwards; set_velocity() gives a
+// body a constant velocity (the v_c of collision_helper, grid.cpp:206-231).
 struct CollisionBody {
     std::function<double(const Vec3&)> phi;
     std::function<Vec3(const Vec3&)> normal;
     std::function<Vec3(const Vec3&)> velocity;
-    double friction;
-    mpm_collider desc;
+    double friction = 0.0;
+    mpm_collider desc{};
+    bool has_desc = false;
 };
 
 // make_plane, collision.hpp:14 / collision.cpp:5-27.
@@ -105,6 +112,50 @@ inline CollisionBody make_plane(const Vec3& point, const Vec3& normal, double fr
     b.velocity = [](const Vec3&) { return Vec3(0, 0, 0); };
     b.friction = friction;
     b.desc = d;
+    b.has_desc = true;
+    return b;
+}
+
+namespace detail {
+inline CollisionBody sdf_body(const mpm_collider& d) {
+    CollisionBody b;
+    b.desc = d;
+    b.has_desc = true;
+    b.friction = d.friction;
+    b.phi = [d](const Vec3& x) {
+        if (d.type == MPM_COLLIDER_SPHERE) {
+            const double r0 = x[0] - d.point[0], r1 = x[1] - d.point[1], r2 = x[2] - d.point[2];
+            return std::sqrt(r0 * r0 + r1 * r1 + r2 * r2) - d.radius;
+        }
+        double q[3], out = 0.0, qmax = -1e300;
+        for (int a = 0; a < 3; ++a) {
+            q[a] = std::fabs(x[a] - d.point[a]) - d.half_extent[a];
+            qmax = std::max(qmax, q[a]);
+            if (q[a] > 0) out += q[a] * q[a];
+        }
+        return std::sqrt(out) + std::min(qmax, 0.0);
+    };
+    b.velocity = [](const Vec3&) { return Vec3(0, 0, 0); };
+    return b;
+}
+}  // namespace detail
+
+// Solid box and sphere bodies (no counterpart in the reference, whose only
+// factory is make_plane).
+inline CollisionBody make_box(const Vec3& centre, const Vec3& half_extent, double friction) {
+    const double c[3] = {centre[0], centre[1], centre[2]}, h[3] = {half_extent[0], half_extent[1], half_extent[2]};
+    return detail::sdf_body(mpm_make_box(c, h, friction));
+}
+inline CollisionBody make_sphere(const Vec3& centre, double radius, double friction) {
+    const double c[3] = {centre[0], centre[1], centre[2]};
+    return detail::sdf_body(mpm_make_sphere(c, radius, friction));
+}
+
+// A constant body velocity: the velocity closure and the device description
+// change together.
+inline CollisionBody& set_velocity(CollisionBody& b, const Vec3& v) {
+    for (int a = 0; a < 3; ++a) b.desc.velocity[a] = v[a];
+    b.velocity = [v](const Vec3&) { return v; };
     return b;
 }
 
@@ -253,12 +304,37 @@ inline void setup(std::vector<Particle>& snow, double grid_spacing) {
     detail::write_back(r, snow);
 }
 
-// step, grid.hpp:97 / grid.cpp:359-450.
-inline void step(std::vector<Particle>& snow, const std::vector<CollisionBody>& colliders,
-                 std::vector<ParticleStencil>& /*stencils*/, Vec3 gravity, double time_step, double grid_spacing,
-                 double mu_0, double lambda_0, double hardening_coefficient, double max_compression,
-                 double max_stretch, double alpha, StepTimings& timings) {
-    if (snow.empty()) return;
+namespace detail {
+// The device description of each body, after checking that it still
+// describes the body (see CollisionBody).
+inline std::vector<mpm_collider> descriptions(const std::vector<CollisionBody>& colliders) {
+    std::vector<mpm_collider> coll;
+    for (const auto& b : colliders) {
+        if (!b.has_desc)
+            throw std::runtime_error("step: a CollisionBody not built by make_plane/make_box/make_sphere "
+                                     "(custom phi/normal closures cannot run on the GPU)");
+        if (b.friction != b.desc.friction)
+            throw std::runtime_error("step: CollisionBody::friction changed after the body was made");
+        if (b.velocity) {
+            const Vec3 probe = b.velocity(Vec3(0, 0, 0));
+            if (probe[0] != b.desc.velocity[0] || probe[1] != b.desc.velocity[1] || probe[2] != b.desc.velocity[2])
+                throw std::runtime_error("step: CollisionBody::velocity changed outside set_velocity()");
+        }
+        coll.push_back(b.desc);
+    }
+    return coll;
+}
+}  // namespace detail
+
+// nsub calls of step() (grid.hpp:97 / grid.cpp:359-450) with the particle
+// vector written back once at the end: the frame loop of a driver that only
+// reads the particles between frames (src/main.cpp:46-57) needs no download
+// per substep.  Same results as nsub calls of step().
+inline void step_substeps(std::vector<Particle>& snow, const std::vector<CollisionBody>& colliders,
+                          std::vector<ParticleStencil>& /*stencils*/, Vec3 gravity, double time_step,
+                          double grid_spacing, double mu_0, double lambda_0, double hardening_coefficient,
+                          double max_compression, double max_stretch, double alpha, StepTimings& timings, int nsub) {
+    if (snow.empty() || nsub <= 0) return;
     auto& r = detail::resident();
     mpm_params prm{};
     for (int a = 0; a < 3; ++a) prm.gravity[a] = gravity[a];
@@ -271,14 +347,13 @@ inline void step(std::vector<Particle>& snow, const std::vector<CollisionBody>& 
     prm.max_stretch = max_stretch;
     prm.alpha = alpha;
     prm.nu = 0.2;
-    std::vector<mpm_collider> coll;
-    for (const auto& b : colliders) coll.push_back(b.desc);
+    const std::vector<mpm_collider> coll = detail::descriptions(colliders);
     const bool resident_ok = r.eng && r.sid >= 0 && detail::same_params(prm, r.prm) && coll.size() == r.coll.size() &&
                              std::memcmp(coll.data(), r.coll.data(), coll.size() * sizeof(mpm_collider)) == 0 &&
                              detail::same(snow, r.shadow);
     if (!resident_ok) detail::upload(r, snow, prm, coll, true);
     mpm_timings t{};
-    detail::check(mpm_step(r.eng, 1, &t), "mpm_step");
+    detail::check(mpm_step(r.eng, nsub, &t), "mpm_step");
     timings.stencil += t.stencil;
     timings.p2g += t.p2g;
     timings.forces_constitutive += t.forces_constitutive;
@@ -290,6 +365,15 @@ inline void step(std::vector<Particle>& snow, const std::vector<CollisionBody>& 
     timings.particle_collisions += t.particle_collisions;
     timings.position_update += t.position_update;
     detail::write_back(r, snow);
+}
+
+// step, grid.hpp:97 / grid.cpp:359-450.
+inline void step(std::vector<Particle>& snow, const std::vector<CollisionBody>& colliders,
+                 std::vector<ParticleStencil>& stencils, Vec3 gravity, double time_step, double grid_spacing,
+                 double mu_0, double lambda_0, double hardening_coefficient, double max_compression,
+                 double max_stretch, double alpha, StepTimings& timings) {
+    step_substeps(snow, colliders, stencils, gravity, time_step, grid_spacing, mu_0, lambda_0, hardening_coefficient,
+                  max_compression, max_stretch, alpha, timings, 1);
 }
 
 // import_particles, import.hpp:25 / import.cpp:131-137.

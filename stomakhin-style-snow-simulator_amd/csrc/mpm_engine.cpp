@@ -86,8 +86,10 @@ int Engine::init(const mpm_config& cfg) {
         hl = 16;
         while (((size_t)1 << hl) < want) ++hl;
     }
-    if (hl > 26) {
-        set_error("hash_log2 must be <= 26");
+    // A cell key is (slot << 6) | cell in 32 bits and 0xFFFFFFFF marks an
+    // unbinned particle, so slots must stay below 2^25 for no key to equal it.
+    if (hl > 25) {
+        set_error("hash_log2 must be <= 25");
         return MPM_ERR_ARG;
     }
     H_ = 1u << hl;
@@ -585,6 +587,10 @@ int mpm_scene_status(mpm_engine* e, int scene, uint32_t* status, uint64_t* fb) {
 
 int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]) {
     return (e && out7) ? e->eng.scene_sums(scene, out7) : MPM_ERR_ARG;
+}
+
+int mpm_scene_steps(mpm_engine* e, int scene, uint64_t* steps) {
+    return (e && steps) ? e->eng.scene_steps(scene, steps) : MPM_ERR_ARG;
 }
 
 int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps) {

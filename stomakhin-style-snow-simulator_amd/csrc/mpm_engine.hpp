@@ -28,7 +28,14 @@ struct SceneHost {
     bool used = false;
     bool pending_drop = false;  // removed; particles leave at the next (permuting) step
     size_t n = 0;
+    uint64_t steps = 0;  // steps taken since the scene was added
     mpm_params prm{};
+};
+
+// Device bytes a scene holds in the engine's three pools.
+struct Footprint {
+    size_t particles, blocks, partials;
+    size_t total() const { return particles + blocks + partials; }
 };
 
 class Engine {
@@ -58,6 +65,16 @@ class Engine {
     int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
     int scene_status(int sid, uint32_t* status, uint64_t* fb);
     int scene_sums(int sid, double out[7]);
+    int scene_steps(int sid, uint64_t* steps) const {
+        if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) {
+            set_error("bad scene id");
+            return MPM_ERR_ARG;
+        }
+        *steps = scenes_[sid].steps;
+        return MPM_OK;
+    }
+    Footprint footprint(size_t n, const double* x, double dx) const;
+    Footprint pools() const;
     void kernel_times(double ms[5], int* nsteps) const {
         for (int j = 0; j < 5; ++j) ms[j] = kernel_ms_[j];
         if (nsteps) *nsteps = kernel_steps_;
@@ -71,6 +88,8 @@ class Engine {
     void advance(int permute) {
         mov_ ^= 1;
         if (permute) con_ ^= 1;
+        for (auto& sh : scenes_)
+            if (sh.used) ++sh.steps;
     }
     int device_count(uint32_t* n);
     int upload_scene_table(int sid);
@@ -78,7 +97,6 @@ class Engine {
     int ensure_graph(int permute);
     bool take_permute();
     void drop_graphs();
-    double footprint_blocks(size_t n) const;
 
     mpm_config cfg_{};
     hipStream_t stream_ = nullptr;

@@ -144,9 +144,61 @@ int Engine::autotune(int spc, int chosen[4]) {
     return check_status();
 }
 
-// Blocks a scene of n particles may occupy (home + neighbour blocks): a
-// conservative bound for compact bodies at 8 particles per cell.
-double Engine::footprint_blocks(size_t n) const { return 32.0 + 3.0 * (double)n / 256.0; }
+// Device bytes per particle slot (both particle buffers with pid and sid,
+// the four sort arrays, the FP64 fallback queue), per grid block (hash key,
+// cell counts and offsets, particle range, neighbour and home-block indices,
+// v/u, list entries) and per home block (the P2G partial sums of its 7^3
+// node region).
+static size_t bytes_per_particle(bool track) {
+    const size_t nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1 + kStressCh;
+    return 2 * (nf * 4 + 8) + 4 * 4 + 4 + 36;
+}
+constexpr size_t kBytesPerBlock = 8 + 2 * 64 * 4 + 4 + 4 + 8 * 4 + 8 * 4 + kVelCh * 64 * 4 + 4 + 4;
+constexpr size_t kBytesPerHome = (size_t)kGridCh * kRegion * 4;
+
+Footprint Engine::pools() const {
+    return Footprint{cfg_.particle_capacity * bytes_per_particle(cfg_.track_fp != 0), (size_t)(H_ / 2) * kBytesPerBlock,
+                     (size_t)grid_.max_home * kBytesPerHome};
+}
+
+// Blocks from the bounding box of the positions in 4^3-node blocks (home
+// block of x: (floor(x/dx) - 1) div 4; a home block's nodes also reach the +1
+// block per axis), with a 25% margin for a body that spreads as it moves.
+Footprint Engine::footprint(size_t n, const double* x, double dx) const {
+    Footprint f{n * bytes_per_particle(cfg_.track_fp != 0), 0, 0};
+    if (n == 0 || !x || !(dx > 0)) return f;
+    long lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+        long b[3];
+        bool finite = true;
+        for (int a = 0; a < 3; ++a) {
+            const double g = x[i * 3 + a] / dx;
+            if (!std::isfinite(g) || std::fabs(g) > 1e9) {
+                finite = false;
+                break;
+            }
+            const long base = (long)std::floor(g) - 1;
+            b[a] = base >= 0 ? base / 4 : -((-base + 3) / 4);
+        }
+        if (!finite) continue;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = any ? std::min(lo[a], b[a]) : b[a];
+            hi[a] = any ? std::max(hi[a], b[a]) : b[a];
+        }
+        any = true;
+    }
+    if (!any) return f;
+    double home = 1.0, all = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        home *= (double)(hi[a] - lo[a] + 1);
+        all *= (double)(hi[a] - lo[a] + 2);
+    }
+    const double cap = 1e15;
+    f.blocks = (size_t)std::min(cap, std::ceil(1.25 * all) + 8.0) * kBytesPerBlock;
+    f.partials = (size_t)std::min(cap, std::ceil(1.25 * home) + 8.0) * kBytesPerHome;
+    return f;
+}
 
 int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_stats* st) {
     if (!R || nreq < 0 || !st) {
@@ -154,9 +206,8 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
         return MPM_ERR_ARG;
     }
     for (int k = 0; k < nreq; ++k)
-        if (R[k].steps < 0 || R[k].ncolliders < 0 ||
-            R[k].ncolliders > (int)(sizeof R[k].colliders / sizeof R[k].colliders[0])) {
-            set_error("mpm_serve: a request has steps < 0 or more colliders than mpm_request holds");
+        if (R[k].steps < 0 || R[k].ncolliders < 0 || R[k].ncolliders > MPM_MAX_COLLIDERS) {
+            set_error("mpm_serve: a request has steps < 0 or more than MPM_MAX_COLLIDERS colliders");
             return MPM_ERR_ARG;
         }
     if (max_chunk <= 0) max_chunk = 25;
@@ -177,9 +228,30 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
     std::vector<int> order(nreq);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return R[a].arrival_s < R[b].arrival_s; });
+    std::vector<Footprint> fp(nreq);
+    for (int k = 0; k < nreq; ++k) {
+        R[k].status = 0;
+        R[k].t_admit_s = R[k].t_done_s = 0.0;
+        fp[k] = footprint(R[k].n, R[k].x, R[k].prm.dx);
+        R[k].footprint_bytes = fp[k].total();
+    }
+    const Footprint pool = pools();
     int rc;
-    double psteps = 0;
-    int completed = 0, max_res = 0;
+    double psteps = 0, max_bytes = 0;
+    int completed = 0, failed = 0, max_res = 0;
+    // Any error other than a scene's own status ends the call with the engine
+    // emptied, so the caller can serve again.
+    auto fail = [&](int code) {
+        const std::string msg = g_last_error;
+        (void)clear();
+        set_error(msg);
+        return code;
+    };
+    auto reject = [&](mpm_request& q) {
+        q.status = MPM_STATUS_REJECTED;
+        q.t_done_s = now();
+        ++failed;
+    };
 
     auto add = [&](mpm_request& q, int* sid) {
         return scene_add(q.prm, q.colliders, q.ncolliders, q.n, q.x, q.v, q.m, nullptr, nullptr, nullptr, q.E, sid);
@@ -189,51 +261,79 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
             mpm_request& q = R[order[k]];
             wait_until(q.arrival_s);
             int sid;
-            if ((rc = clear())) return rc;
-            if ((rc = add(q, &sid))) return rc;
+            if ((rc = clear())) return fail(rc);
+            if (q.n > cfg_.particle_capacity) {
+                reject(q);
+                continue;
+            }
+            if ((rc = add(q, &sid))) return fail(rc);
             q.t_admit_s = now();
-            if ((rc = step(q.steps, nullptr))) return rc;
+            max_bytes = std::max(max_bytes, (double)fp[order[k]].total());
+            rc = step(q.steps, nullptr);
+            if (rc == MPM_ERR_DEVICE) {
+                uint32_t bits = 0;
+                if ((rc = scene_status(sid, &bits, nullptr))) return fail(rc);
+                q.status = bits ? bits : MPM_STATUS_NAN;
+                q.t_done_s = now();
+                ++failed;
+                continue;
+            }
+            if (rc) return fail(rc);
             if (q.out_x || q.out_v)
-                if ((rc = scene_download(sid, q.out_x, q.out_v, nullptr, nullptr, nullptr, nullptr))) return rc;
+                if ((rc = scene_download(sid, q.out_x, q.out_v, nullptr, nullptr, nullptr, nullptr))) return fail(rc);
             q.t_done_s = now();
             psteps += (double)q.n * q.steps;
             ++completed;
             max_res = 1;
         }
-        if ((rc = clear())) return rc;
+        if ((rc = clear())) return fail(rc);
     } else {
         struct Act {
             int req, sid, left;
         };
         std::vector<Act> act;
         size_t next = 0;
-        size_t res_n = 0, pend_n = 0;
-        double res_blocks = 0;
-        const double block_budget = 0.5 * (double)H_;
-        while (completed < nreq) {
+        // bytes reserved in each pool by resident requests, and particle
+        // slots of retired scenes not yet compacted away
+        Footprint res{0, 0, 0};
+        size_t pend = 0;
+        auto retire = [&](size_t i) {
+            const Footprint& f = fp[act[i].req];
+            res.particles -= f.particles;
+            res.blocks -= f.blocks;
+            res.partials -= f.partials;
+            pend += f.particles;
+            act.erase(act.begin() + (long)i);
+        };
+        while (completed + failed < nreq) {
             const double tn = now();
             while (next < (size_t)nreq && R[order[next]].arrival_s <= tn) {
-                mpm_request& q = R[order[next]];
-                const double fb = footprint_blocks(q.n);
-                bool fits = res_n + pend_n + q.n <= cfg_.particle_capacity &&
-                            (act.empty() || res_blocks + fb <= block_budget);
+                const int r = order[next];
+                mpm_request& q = R[r];
+                const Footprint& f = fp[r];
+                if (f.particles > pool.particles) {  // can never fit
+                    reject(q);
+                    ++next;
+                    continue;
+                }
+                bool fits = res.particles + pend + f.particles <= pool.particles &&
+                            res.blocks + f.blocks <= pool.blocks && res.partials + f.partials <= pool.partials;
                 if (!fits && act.empty()) {
-                    // Nothing resident: drop the retired scenes' particles now.
-                    if ((rc = clear())) return rc;
-                    pend_n = 0;
-                    fits = q.n <= cfg_.particle_capacity;
-                    if (!fits) {
-                        set_error("mpm_serve: a request is larger than the particle capacity");
-                        return MPM_ERR_CAPACITY;
-                    }
+                    // Nothing resident: drop the retired scenes' particles now and
+                    // admit the request alone (its block estimate is an upper bound).
+                    if ((rc = clear())) return fail(rc);
+                    pend = 0;
+                    fits = true;
                 }
                 if (!fits) break;
                 int sid;
-                if ((rc = add(q, &sid))) return rc;
+                if ((rc = add(q, &sid))) return fail(rc);
                 q.t_admit_s = now();
-                act.push_back({order[next], sid, q.steps});
-                res_n += q.n;
-                res_blocks += fb;
+                act.push_back({r, sid, q.steps});
+                res.particles += f.particles;
+                res.blocks += f.blocks;
+                res.partials += f.partials;
+                max_bytes = std::max(max_bytes, (double)res.total());
                 ++next;
             }
             if (act.empty()) {
@@ -243,37 +343,53 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
             max_res = std::max(max_res, (int)act.size());
             int K = max_chunk;
             for (const auto& a : act) K = std::min(K, a.left);
-            if ((rc = step_graph(K))) return rc;
-            pend_n = 0;
+            rc = step_graph(K);
+            if (rc && rc != MPM_ERR_DEVICE) return fail(rc);
+            pend = 0;
+            size_t res_n = 0;
+            for (const auto& a : act) res_n += R[a.req].n;
             psteps += (double)K * (double)res_n;
             for (size_t i = 0; i < act.size();) {
+                mpm_request& q = R[act[i].req];
+                uint32_t bits = 0;
+                if (rc == MPM_ERR_DEVICE) {
+                    int rs = scene_status(act[i].sid, &bits, nullptr);
+                    if (rs) return fail(rs);
+                }
                 act[i].left -= K;
+                if (bits) {  // this request's scene failed: retire it, keep the others
+                    q.status = bits;
+                    if ((rc = scene_remove(act[i].sid))) return fail(rc);
+                    q.t_done_s = now();
+                    ++failed;
+                    retire(i);
+                    continue;
+                }
                 if (act[i].left > 0) {
                     ++i;
                     continue;
                 }
-                mpm_request& q = R[act[i].req];
                 if (q.out_x || q.out_v)
                     if ((rc = scene_download(act[i].sid, q.out_x, q.out_v, nullptr, nullptr, nullptr, nullptr)))
-                        return rc;
-                if ((rc = scene_remove(act[i].sid))) return rc;
+                        return fail(rc);
+                if ((rc = scene_remove(act[i].sid))) return fail(rc);
                 q.t_done_s = now();
-                res_n -= q.n;
-                pend_n += q.n;
-                res_blocks -= footprint_blocks(q.n);
                 ++completed;
-                act.erase(act.begin() + (long)i);
+                retire(i);
             }
         }
-        if ((rc = clear())) return rc;
+        if ((rc = clear())) return fail(rc);
     }
     st->wall_s = now();
     st->completed = completed;
+    st->failed = failed;
     st->max_resident = max_res;
+    st->max_resident_bytes = max_bytes;
     st->particle_steps = psteps;
     st->scenes_per_s = st->wall_s > 0 ? completed / st->wall_s : 0.0;
     std::vector<double> lat;
-    for (int k = 0; k < nreq; ++k) lat.push_back(R[k].t_done_s - R[k].arrival_s);
+    for (int k = 0; k < nreq; ++k)
+        if (R[k].status == 0) lat.push_back(R[k].t_done_s - R[k].arrival_s);
     std::sort(lat.begin(), lat.end());
     if (!lat.empty()) {
         auto pct = [&](double p) {
@@ -309,6 +425,18 @@ int mpm_permute_period(mpm_engine* e, int* period) {
 
 int mpm_serve(mpm_engine* e, mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* stats) {
     return e ? e->eng.serve(reqs, nreq, mode, max_chunk, stats) : MPM_ERR_ARG;
+}
+
+int mpm_request_footprint(mpm_engine* e, const mpm_request* q, size_t* bytes, size_t pools[3]) {
+    if (!e || !q || !bytes) return MPM_ERR_ARG;
+    *bytes = e->eng.footprint(q->n, q->x, q->prm.dx).total();
+    if (pools) {
+        const snowmpm::Footprint p = e->eng.pools();
+        pools[0] = p.particles;
+        pools[1] = p.blocks;
+        pools[2] = p.partials;
+    }
+    return MPM_OK;
 }
 
 }  // extern "C"

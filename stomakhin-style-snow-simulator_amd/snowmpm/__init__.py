@@ -24,6 +24,8 @@ lib = C.CDLL(_LIB)
 OK = 0
 ERR_ARG, ERR_HIP, ERR_OOM, ERR_CAPACITY, ERR_STATE, ERR_DEVICE, ERR_IO = -1, -2, -3, -4, -5, -6, -7
 STATUS_NAN, STATUS_OVERFLOW, STATUS_HASH_FULL, STATUS_RANGE = 0x1, 0x2, 0x4, 0x8
+STATUS_REJECTED = 0x100  # serving: a request larger than the engine's particle pool
+MAX_COLLIDERS = 8
 PLANE, BOX, SPHERE = 0, 1, 2
 PROJECT = 0x1  # Collider.flags: project particles that end a step inside the body onto its surface
 
@@ -69,15 +71,17 @@ class Request(C.Structure):
     """mpm_request: one scene request for the serving loop."""
     _fields_ = [("arrival_s", C.c_double), ("steps", C.c_int), ("ncolliders", C.c_int), ("n", C.c_size_t),
                 ("x", C.POINTER(C.c_double)), ("v", C.POINTER(C.c_double)), ("m", C.POINTER(C.c_double)),
-                ("E", C.POINTER(C.c_double)), ("prm", Params), ("colliders", Collider * 4),
+                ("E", C.POINTER(C.c_double)), ("prm", Params), ("colliders", Collider * MAX_COLLIDERS),
                 ("out_x", C.POINTER(C.c_double)), ("out_v", C.POINTER(C.c_double)),
-                ("t_admit_s", C.c_double), ("t_done_s", C.c_double)]
+                ("t_admit_s", C.c_double), ("t_done_s", C.c_double), ("status", C.c_uint32),
+                ("footprint_bytes", C.c_size_t)]
 
 
 class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("scenes_per_s", C.c_double), ("p50_s", C.c_double),
                 ("p99_s", C.c_double), ("mean_s", C.c_double), ("max_s", C.c_double),
-                ("particle_steps", C.c_double), ("completed", C.c_int), ("max_resident", C.c_int)]
+                ("particle_steps", C.c_double), ("completed", C.c_int), ("max_resident", C.c_int),
+                ("failed", C.c_int), ("max_resident_bytes", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -100,6 +104,9 @@ _SIGS = {
     "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
+    "mpm_scene_steps": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
+    "mpm_request_footprint": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_size_t),
+                                        C.POINTER(C.c_size_t)]),
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mpm_permute_period": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -152,24 +159,28 @@ def _v3(x):
     return (C.c_double * 3)(*map(float, x))
 
 
-def _flagged(c, project):
+def _flagged(c, project, velocity):
     if project:
         c.flags |= PROJECT
+    if velocity is not None:
+        c.velocity[:] = list(map(float, velocity))
     return c
 
 
-def make_plane(point, normal, friction, project=False):
-    """make_plane (collision.hpp:14, collision.cpp:5-27).  project=True adds
-    the position projection, which the reference does not have."""
-    return _flagged(lib.mpm_make_plane(_v3(point), _v3(normal), float(friction)), project)
+def make_plane(point, normal, friction, project=False, velocity=None):
+    """make_plane (collision.hpp:14, collision.cpp:5-27).  velocity sets the
+    body's constant velocity (CollisionBody::velocity, collision.hpp:10,
+    which make_plane leaves at zero); project=True adds the position
+    projection, which the reference does not have."""
+    return _flagged(lib.mpm_make_plane(_v3(point), _v3(normal), float(friction)), project, velocity)
 
 
-def make_box(centre, half_extent, friction, project=False):
-    return _flagged(lib.mpm_make_box(_v3(centre), _v3(half_extent), float(friction)), project)
+def make_box(centre, half_extent, friction, project=False, velocity=None):
+    return _flagged(lib.mpm_make_box(_v3(centre), _v3(half_extent), float(friction)), project, velocity)
 
 
-def make_sphere(centre, radius, friction, project=False):
-    return _flagged(lib.mpm_make_sphere(_v3(centre), float(radius), float(friction)), project)
+def make_sphere(centre, radius, friction, project=False, velocity=None):
+    return _flagged(lib.mpm_make_sphere(_v3(centre), float(radius), float(friction)), project, velocity)
 
 
 def lame(E, nu):
@@ -320,26 +331,26 @@ class Engine:
         _check(lib.mpm_permute_period(self.h, C.byref(p)))
         return p.value
 
-    def serve(self, requests, mode=0, max_chunk=25, keep_outputs=False):
-        """Runs the request loop.  requests: dicts with arrival_s, steps and a
-        generate_scene() payload (x, v, m, E, params, colliders).  mode 0 is
-        continuous batching, mode 1 the plain per-scene step one at a time."""
-        arr = (Request * len(requests))()
+    @staticmethod
+    def _requests(requests, keep_outputs=False):
+        arr = (Request * max(1, len(requests)))()
         keep = []
         outs = []
         for k, q in enumerate(requests):
             r = arr[k]
-            r.arrival_s = float(q["arrival_s"])
-            r.steps = int(q["steps"])
+            r.arrival_s = float(q.get("arrival_s", 0.0))
+            r.steps = int(q.get("steps", 0))
             x = np.ascontiguousarray(q["x"], dtype=np.float64)
             v = np.ascontiguousarray(q["v"], dtype=np.float64)
-            m = np.ascontiguousarray(q["m"], dtype=np.float64)
+            m = np.ascontiguousarray(np.broadcast_to(np.asarray(q["m"], dtype=np.float64), (x.shape[0],)))
             E = None if q.get("E") is None else np.ascontiguousarray(q["E"], dtype=np.float64)
             keep += [x, v, m, E]
             r.n = x.shape[0]
             r.x, r.v, r.m, r.E = _dptr(x), _dptr(v), _dptr(m), _dptr(E)
             r.prm = q["params"]
             cs = q["colliders"]
+            if len(cs) > MAX_COLLIDERS:
+                raise MpmError(ERR_ARG, f"a request holds at most {MAX_COLLIDERS} colliders")
             r.ncolliders = len(cs)
             for j, c in enumerate(cs):
                 r.colliders[j] = c
@@ -351,13 +362,41 @@ class Engine:
                 r.out_x, r.out_v = _dptr(ox), _dptr(ov)
             else:
                 outs.append(None)
+        return arr, keep, outs
+
+    def footprint(self, request):
+        """Device bytes a request would reserve, and the engine's pools
+        (particles, blocks, partial sums) in bytes."""
+        arr, keep, _ = self._requests([request])
+        b = C.c_size_t()
+        pools = (C.c_size_t * 3)()
+        _check(lib.mpm_request_footprint(self.h, arr, C.byref(b), pools))
+        return b.value, tuple(pools)
+
+    def serve(self, requests, mode=0, max_chunk=25, keep_outputs=False):
+        """Runs the request loop.  requests: dicts with arrival_s, steps and a
+        generate_scene() payload (x, v, m, E, params, colliders).  mode 0 is
+        continuous batching, mode 1 the plain per-scene step one at a time.
+        Per request the result lists the latency, the status (0, or the
+        device status bits that retired it) and the reserved footprint."""
+        arr, keep, outs = self._requests(requests, keep_outputs)
         st = ServeStats()
         _check(lib.mpm_serve(self.h, arr, len(requests), int(mode), int(max_chunk), C.byref(st)))
         res = st.as_dict()
         res["latency_s"] = [arr[k].t_done_s - arr[k].arrival_s for k in range(len(requests))]
+        res["status"] = [arr[k].status for k in range(len(requests))]
+        res["footprint_bytes"] = [arr[k].footprint_bytes for k in range(len(requests))]
+        res["admit_s"] = [arr[k].t_admit_s for k in range(len(requests))]
+        res["done_s"] = [arr[k].t_done_s for k in range(len(requests))]
         if keep_outputs:
             res["outputs"] = outs
         return res
+
+    def steps(self, sid):
+        """Steps the scene has taken since it was added."""
+        n = C.c_uint64()
+        _check(lib.mpm_scene_steps(self.h, int(sid), C.byref(n)))
+        return n.value
 
     def step_rc(self, nsteps=1):
         """Like step() but returns the status code instead of raising."""

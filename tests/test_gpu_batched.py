@@ -29,25 +29,22 @@ def test_graph_steps_equal_plain_steps():
 
 
 def test_autotune_keeps_results():
+    """The autotuner times ordinary steps (launch-shape candidates, then 4
+    re-sort periods x 16 steps); the scene's state after them equals, bit for
+    bit, the same number of plain steps."""
     sc = snowmpm.generate_scene(0, seed=3, scale=0.3)
     with snowmpm.Engine(capacity=len(sc["x"]) + 64) as eng:
         sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
         eng.setup()
         chosen = eng.autotune(1)
         assert all(v > 0 for v in chosen.values())
-        # the tuner ran ordinary steps: count them from a reference run
+        nsteps = eng.steps(sid)
         got = eng.download(sid)
     assert chosen["permute_period"] in (2, 4, 8, 16)
-    nsteps_tuner = None
-    with snowmpm.Engine(capacity=len(sc["x"]) + 64) as ref_eng:
-        rid = ref_eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
-        for n in range(1, 200):
-            ref_eng.step(1)
-            if np.array_equal(ref_eng.download(rid, fields=("x",))["x"], got["x"]):
-                nsteps_tuner = n
-                break
-    # launch-shape candidates (one step each) and 4 re-sort periods x 16 steps
-    assert nsteps_tuner is not None and nsteps_tuner > 64
+    assert nsteps > 64
+    ref = mpmref.run_gpu(sc, nsteps)
+    for k in ("x", "v", "Fe", "Jp"):
+        assert np.array_equal(ref[k], got[k]), k
 
 
 def test_served_results_equal_alone():

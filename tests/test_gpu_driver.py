@@ -1,11 +1,13 @@
-"""Driver parity: the compat snow_sim binary (the reference main.cpp call
-sequence, src/main.cpp:12-59, compiled against compat/snow_compat.hpp and the
-product library) reads groups.csv / particles.csv, runs setup and frames of
-substeps on the GPU and writes frame CSVs; the frames must match the FP64
-oracle run with main.cpp's constants (dx 0.05, dt 5e-4, E 2.5e5, nu 0.2,
-xi 10, theta_c 0.015, theta_s 0.0075, alpha 0.95, plane y = 0 with
-friction 1, g = (0, -10, 0)).  Frame rows are x, -z, y printed with 6
-significant digits, so the bound is 1e-4 relative plus the print rounding."""
+"""Driver parity: the compat snow_sim binary (compiled against
+compat/snow_compat.hpp and the product library; the API call order of the
+reference's src/main.cpp:12-59) reads groups.csv / particles.csv, runs setup
+and frames of substeps on the GPU and writes frame CSVs; the frames must
+match the FP64 oracle run with the same settings: the defaults (the scene
+main.cpp hard-codes: dx 0.05, dt 5e-4, E 2.5e5, nu 0.2, xi 10, theta_c
+0.015, theta_s 0.0075, alpha 0.95, plane y = 0 with friction 1,
+g = (0, -10, 0)), and a scene file with other material settings and a
+moving plane.  Frame rows are x, -z, y printed with 6 significant digits, so
+the bound is 1e-4 relative plus the print rounding."""
 import os
 import subprocess
 
@@ -26,8 +28,7 @@ def _frame(path):
     return np.stack([rows[:, 0], rows[:, 2], -rows[:, 1]], axis=1)  # back to solver axes
 
 
-def test_snow_sim_frames_match_oracle(tmp_path):
-    assert os.path.exists(SIM), "build the compat driver first (__graft_entry__.build())"
+def _inputs(tmp_path):
     h = 0.025
     gi = np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)
     x = (gi + 0.5) * h + np.array([-0.1, 0.01, -0.1])
@@ -37,20 +38,15 @@ def test_snow_sim_frames_match_oracle(tmp_path):
         f.write("group_id,x,y,z\n")
         for p in x:
             f.write(f"4,{p[0]:.17g},{p[1]:.17g},{p[2]:.17g}\n")
-    frames, sub = 2, 5
-    out = tmp_path / "output"
-    r = subprocess.run([SIM, str(tmp_path / "groups.csv"), str(tmp_path / "particles.csv"), str(frames), str(sub),
-                        str(out)], capture_output=True, text=True, timeout=120)
-    print(r.stdout[-400:], r.stderr[-400:])
-    assert r.returncode == 0
-    assert f"{len(x)} particles" in r.stdout
+    return x
 
-    E, nu = 2.5e5, 0.2
+
+def _check_frames(out, x, frames, sub, E, nu, xi, tc, ts, dt, dx, g, planes):
     mu0, la0 = E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
     st = orc.State(x, np.tile([0.3, -1.5, 0.0], (len(x), 1)), np.full(len(x), 0.05))
-    st.setup(0.05)
-    op = orc.params(5e-4, 0.05, mu0, la0, 10.0, 0.015, 0.0075, 0.95, gravity=(0, -10, 0))
-    cs = [orc.make_plane((0, 0, 0), (0, 1, 0), 1.0)]
+    st.setup(dx)
+    op = orc.params(dt, dx, mu0, la0, xi, tc, ts, 0.95, gravity=g)
+    cs = [orc.make_plane(p[:3], p[3:6], p[6], velocity=p[7:10] if len(p) == 10 else None) for p in planes]
     f0 = _frame(out / "frame_000000.csv")
     assert np.abs(f0 - x).max() <= 1e-6 * np.abs(x).max()
     L = float(np.ptp(x, axis=0).max())
@@ -61,3 +57,37 @@ def test_snow_sim_frames_match_oracle(tmp_path):
         err = np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), L)
         print(f"frame {k}: max rel err {err.max():.2e}")
         assert err.max() < 1e-4
+
+
+def test_snow_sim_frames_match_oracle(tmp_path):
+    assert os.path.exists(SIM), "build the compat driver first (__graft_entry__.build())"
+    x = _inputs(tmp_path)
+    frames, sub = 2, 5
+    out = tmp_path / "output"
+    r = subprocess.run([SIM, "--groups", str(tmp_path / "groups.csv"), "--particles", str(tmp_path / "particles.csv"),
+                        "--frames", str(frames), "--substeps", str(sub), "--out", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-400:], r.stderr[-400:])
+    assert r.returncode == 0
+    assert f"loaded {len(x)} particles" in r.stdout
+    _check_frames(out, x, frames, sub, 2.5e5, 0.2, 10.0, 0.015, 0.0075, 5e-4, 0.05, (0, -10, 0),
+                  [(0, 0, 0, 0, 1, 0, 1.0)])
+
+
+def test_snow_sim_scene_file_with_moving_plane(tmp_path):
+    """Settings from a scene file (material, dx, dt, gravity) and two planes,
+    the ground moving as a conveyor (v = (0.8, 0, -0.2)) and a tilted wall."""
+    x = _inputs(tmp_path)
+    frames, sub = 3, 8
+    out = tmp_path / "frames"
+    planes = [(0, 0, 0, 0, 1, 0, 0.6, 0.8, 0.0, -0.2), (0.09, 0, 0, -1, 1, 0, 0.3)]
+    lines = [f"groups {tmp_path / 'groups.csv'}", f"particles {tmp_path / 'particles.csv'}", f"frames {frames}",
+             f"substeps {sub}", f"out {out}", "dx 0.04", "dt 2e-4", "E 1.4e5", "nu 0.25", "xi 8",
+             "theta-c 0.025  # compression", "theta-s 0.005", "gravity 0,-9.81,0"]
+    lines += ["plane " + ",".join(str(v) for v in p) for p in planes]
+    (tmp_path / "scene.txt").write_text("\n".join(lines) + "\n")
+    r = subprocess.run([SIM, "--scene", str(tmp_path / "scene.txt"), "--quiet"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-400:], r.stderr[-400:])
+    assert r.returncode == 0
+    _check_frames(out, x, frames, sub, 1.4e5, 0.25, 8.0, 0.025, 0.005, 2e-4, 0.04, (0, -9.81, 0), planes)

@@ -47,7 +47,7 @@ def test_long_run_aggregates_match_oracle():
     e_P = np.abs(sg["P"] - sr["P"]).max() / (sr["M"] * 2.0)
     e_hist = np.abs(sg["hist"] - sr["hist"]).sum() / 2
     e_comp = abs(sg["comp"] - sr["comp"]) / sr["comp"]
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, steps)
     print(f"200 steps: com {e_com:.2e} momentum {e_P:.2e} J_P hist {e_hist:.2e} "
           f"compacted {e_comp:.2e} (yielded {sg['yielded']} / {sr['yielded']}) x {ex:.2e} v {ev:.2e}")
     assert sr["yielded"] > len(sc["x"]) // 2          # the run does compact the snow

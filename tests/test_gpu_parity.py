@@ -1,7 +1,9 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
 
-Tolerances: per-particle relative position and velocity error <= 1e-4 after
-10 steps (FP32 GPU vs FP64 oracle), as the metric in mpmref.rel_errors.
+Tolerances: 1e-4 (FP32 GPU vs FP64 oracle) on the metrics of
+mpmref.rel_errors: velocity error relative to the scene's largest speed, and
+position error beyond FP32 storage rounding relative to the scene's largest
+displacement; after 10 steps, and after 20-30 steps for the contact cases.
 Bitwise equality where the design guarantees it (repeat runs, permuted
 input order, a scene alone vs in a batch).
 """
@@ -64,8 +66,9 @@ def test_ten_steps_match_oracle(kind, scale, index):
     sc = _scene(kind, scale, index=index)
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
-    ex, ev = mpmref.rel_errors(got, ref, sc)
-    print(f"kind {kind} n={len(sc['x'])}: rel err x {ex:.3e} v {ev:.3e}")
+    ex, ev = mpmref.rel_errors(got, ref, sc, 10)
+    used = mpmref.x_rounding_used(got, ref, sc, 10)
+    print(f"kind {kind} n={len(sc['x'])}: rel err x {ex:.3e} v {ev:.3e} (x error {used:.2f} of its FP32 allowance)")
     assert ex < TOL and ev < TOL
     jp_ref = np.linalg.det(ref["Fp"])
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-4
@@ -80,7 +83,7 @@ def test_full_size_tori_ten_steps():
     assert len(sc["x"]) == 1365824
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, 10)
     print(f"tori n={len(sc['x'])}: rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert got["status"][0] == 0
@@ -97,7 +100,7 @@ def test_full_size_slab_ten_steps():
     assert len(sc["x"]) == 8000000
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, 10)
     print(f"slab n={len(sc['x'])}: rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert got["status"][0] == 0
@@ -121,7 +124,7 @@ def test_random_density_matches_oracle(seed, n, side):
     assert counts.max() > 16, counts.max()  # the long-cell path is reached
     ref = mpmref.run_oracle(sc, 10)
     got = mpmref.run_gpu(sc, 10)
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, 10)
     print(f"random density n={n}, max {counts.max()} per cell: rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert np.abs(got["V0"] - ref["V0"]).max() / ref["V0"].max() < 1e-5
@@ -138,9 +141,9 @@ def test_contact_plasticity_matches_oracle():
     got = mpmref.run_gpu(sc, nsteps)
     jp_ref = np.linalg.det(ref["Fp"])
     assert (jp_ref < 0.999).any(), "scenario must plastify"
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, nsteps)
     print(f"contact: rel err x {ex:.3e} v {ev:.3e}, min Jp {jp_ref.min():.4f}")
-    assert ex < TOL and ev < 1e-3
+    assert ex < TOL and ev < TOL
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
@@ -160,9 +163,9 @@ def test_eight_colliders_last_one_hit():
     got = mpmref.run_gpu(sc, nsteps)
     jp_ref = np.linalg.det(ref["Fp"])
     assert (jp_ref < 0.999).any(), "scenario must plastify"
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, nsteps)
     print(f"eight colliders: rel err x {ex:.3e} v {ev:.3e}")
-    assert ex < TOL and ev < 1e-3
+    assert ex < TOL and ev < TOL
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
@@ -179,7 +182,7 @@ def test_sparse_blocks_exceed_the_grid():
               colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
     ref = mpmref.run_oracle(sc, 5)
     got = mpmref.run_gpu(sc, 5)
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, 5)
     print(f"sparse: {len(x)} particles, rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert got["status"][0] == 0
@@ -197,9 +200,9 @@ def test_slab_contact_matches_oracle():
     got = mpmref.run_gpu(sc, nsteps)
     jp_ref = np.linalg.det(ref["Fp"])
     assert (jp_ref < 0.999).any(), "scenario must plastify"
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, nsteps)
     print(f"slab contact: rel err x {ex:.3e} v {ev:.3e}, min Jp {jp_ref.min():.4f}")
-    assert ex < TOL and ev < 1e-3
+    assert ex < TOL and ev < TOL
     assert np.abs(got["Jp"] - jp_ref).max() < 1e-3
 
 
@@ -231,7 +234,7 @@ def test_position_projection_matches_oracle():
     assert depth(ref_plain["x"]).min() < -3e-3, "without projection particles must end inside"
     assert depth(ref["x"]).min() > 0.0
     assert depth(got["x"]).min() > -1e-6
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, nsteps)
     print(f"projection: rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert got["status"][0] == 0
@@ -399,7 +402,7 @@ def test_fp64_fallback_matches_oracle():
     got = mpmref.run_gpu(sc, 3, Fe=Fe)
     _, fb = got["status"]
     assert fb >= n
-    ex, ev = mpmref.rel_errors(got, ref, sc)
+    ex, ev = mpmref.rel_errors(got, ref, sc, 3)
     print(f"fallbacks {fb}: rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert np.abs(got["Fe"] - ref["Fe"]).max() < 1e-4
@@ -441,7 +444,7 @@ def test_golden_trajectory(track_fp):
               colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2),
                          snowmpm.make_plane((0.09, 0, 0), (-s, s, 0), 0.5)])
     got = mpmref.run_gpu(sc, 10, track_fp=track_fp)
-    ex, ev = mpmref.rel_errors(got, dict(x=ref["x_10"], v=ref["v_10"]), sc)
+    ex, ev = mpmref.rel_errors(got, dict(x=ref["x_10"], v=ref["v_10"]), sc, 10)
     print(f"golden trajectory (track_fp={track_fp}): rel err x {ex:.3e} v {ev:.3e}")
     assert ex < TOL and ev < TOL
     assert np.abs(got["V0"] - ref["V0"]).max() / ref["V0"].max() < 1e-5

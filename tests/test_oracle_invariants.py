@@ -289,3 +289,143 @@ def test_position_projection():
     p = o.make_particle((0.3, 0.05, 0.0), (0, 0, 0), 1, 1)
     o.position_projection(p, [plane])
     assert p.x[:] == [0.3, 0.05, 0.0]
+
+
+# ---- whole-step closed forms under contact (pin the order of grid.cpp:402-446) ----
+
+def _block(n=6, h=0.01, lo=(0.013, 0.101, -0.047)):
+    gi = np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    return (gi + 0.5) * h + np.array(lo)
+
+
+@pytest.mark.parametrize("vc", [(0.0, 0.0, 0.0), (0.7, 0.0, -0.3)])
+def test_step_inside_sticky_halfspace_closed_form(vc):
+    """A block with uniform v0 wholly inside a friction-1 half-space (every
+    particle and every stencil node below the plane), pressing into it.
+    After one step: every node sticks to the body (v_i = v_c), so grad v = 0
+    and F_E stays I; G2P gives PIC = v_c and FLIP = v0 + sum w (v_c - v0) =
+    v_c; the particle collision sees v - v_c = 0; x advances by dt v_c.
+    Reading the grid velocity before grid_collisions (S14) would leave
+    v = v0 + dt g; advecting before collisions would move x by dt v0."""
+    x = _block()
+    v0 = np.array([0.2, -3.0, 0.1])
+    st = o.State(x, np.tile(v0, (len(x), 1)), 4e-4)
+    st.setup(0.02)
+    prm = o.params(1e-4, 0.02, *o.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    cs = [o.make_plane((0, 0.5, 0), (0, 1, 0), 1.0, velocity=vc)]
+    st.step(cs, prm)
+    assert np.abs(st.field("v") - np.array(vc)).max() < 1e-12
+    assert np.abs(st.field("x") - (x + prm.dt * np.array(vc))).max() < 1e-14
+    assert np.abs(st.field("Fe") - np.eye(3).reshape(9)).max() < 1e-12
+
+
+def test_step_inside_frictionless_halfspace_closed_form():
+    """The same block in a frictionless moving half-space: grid_collisions
+    removes only the normal part of v - v_c (grid.cpp:213-230), so every
+    particle leaves the step with v = v_c + tangential part of
+    (v0 + dt g - v_c) = (v0_x, v_c,y, v0_z), and x advances by dt times that."""
+    x = _block()
+    v0 = np.array([0.2, -3.0, 0.1])
+    vc = np.array([0.7, 0.05, -0.3])
+    st = o.State(x, np.tile(v0, (len(x), 1)), 4e-4)
+    st.setup(0.02)
+    prm = o.params(1e-4, 0.02, *o.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    st.step([o.make_plane((0, 0.5, 0), (0, 1, 0), 0.0, velocity=vc)], prm)
+    want = np.array([v0[0], vc[1], v0[2]])
+    assert np.abs(st.field("v") - want).max() < 1e-12
+    assert np.abs(st.field("x") - (x + prm.dt * want)).max() < 1e-14
+
+
+def thin_box_case():
+    """Particles around a moving thin box (y in [0.106, 0.114]) that no grid
+    node (y = 0.02 j) lies in, so only particle_collisions sees it.  Groups:
+    A inside near the top face (outward normal +y), B inside near the bottom
+    face (normal -y), C 0.1 mm above the top face (outside before advection,
+    inside after), D well outside.  Returns positions, group labels, v0, v_c
+    and the collider arguments."""
+    xs = np.array([0.013, 0.021, 0.034, 0.046])
+    zs = np.array([-0.031, -0.012, 0.007])
+    ys = {"A": 0.1135, "B": 0.1066, "C": 0.1141, "D": 0.131}
+    pts, lab = [], []
+    for g, y in ys.items():
+        for a in xs:
+            for c in zs:
+                pts.append((a, y, c))
+                lab.append(g)
+    return (np.array(pts), np.array(lab), np.array([0.3, -3.0, 0.2]), np.array([0.5, 0.0, 0.0]),
+            ((0.0, 0.11, 0.0), (1.0, 0.004, 1.0), 0.3))
+
+
+def thin_box_expected(x, lab, v0, vc, dt, g):
+    """Closed form of one step for thin_box_case: the grid sees no collider,
+    so G2P returns v0 + dt g for every particle; particle_collisions at the
+    pre-advection x (grid.cpp:327-333) sticks group A to the box (|v_t| =
+    0.28 <= 0.3 * 3) and leaves B (separating from the bottom face), C (not
+    yet inside) and D alone; advection uses the collided v (grid.cpp:335-338)."""
+    v1 = np.tile(v0 + dt * np.asarray(g), (len(x), 1))
+    v1[lab == "A"] = vc
+    return x + dt * v1, v1
+
+
+def test_step_particle_collision_order_closed_form():
+    x, lab, v0, vc, (c, he, mu) = thin_box_case()
+    st = o.State(x, np.tile(v0, (len(x), 1)), 4e-4)
+    st.setup(0.02)
+    prm = o.params(1e-4, 0.02, *o.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    st.step([o.make_box(c, he, mu, velocity=vc)], prm)
+    xw, vw = thin_box_expected(x, lab, v0, vc, prm.dt, (0, -9.81, 0))
+    assert np.abs(st.field("v") - vw).max() < 1e-12
+    assert np.abs(st.field("x") - xw).max() < 1e-14
+    # group C ends inside the box: colliding after advection would have caught it
+    assert (np.abs(st.field("x")[lab == "C"][:, 1] - 0.11) < 0.004).all()
+
+
+def grid_contact_case():
+    """A block whose particles all lie above a sticky moving plane y = 0.105
+    while the two lowest node layers of its stencils (y = 0.08, 0.10) lie in
+    it: only grid_collisions sees the plane.  Returns x, v0, v_c and the
+    plane's arguments."""
+    return _block(), np.array([0.2, -1.0, 0.1]), np.array([0.7, 0.0, -0.3]), ((0, 0.105, 0), (0, 1, 0), 1.0)
+
+
+def grid_contact_expected(x, v0, vc, dt, dx, g, Y0, N, N_dash):
+    """Closed form of one step for grid_contact_case.  Every node has
+    p/m = v0 and zero force (F_E = I), so after grid_velocity v_i = v0 + dt g;
+    grid_collisions sticks the nodes with y_i <= Y0 to the plane (v_i = v_c).
+    With D = v_c - v0 - dt g and the x/z weights summing to 1:
+      G2P: PIC = FLIP = sum_i w v_i = v0 + dt g + W D,  W = sum_{y_i <= Y0} wy
+      stress_update: grad v = D (0, G, 0)^T,  G = sum_{y_i <= Y0} N'_y / dx
+    (sum_x dwx = sum_z dwz = 0), so F_E = I + dt D (0, G, 0)^T (no clamp:
+    checked by the caller).  The particles lie above the plane, so
+    particle_collisions leaves v, and x advances by dt v."""
+    gy = x[:, 1] / dx
+    base = np.floor(gy).astype(int) - 1
+    W = np.zeros(len(x))
+    G = np.zeros(len(x))
+    for k in range(4):
+        j = base + k
+        inside = j * dx <= Y0
+        W += np.where(inside, [N(a, b) for a, b in zip(j, gy)], 0.0)
+        G += np.where(inside, [N_dash(a, b) for a, b in zip(j, gy)], 0.0) / dx
+    D = vc - v0 - dt * np.asarray(g)
+    v1 = v0 + dt * np.asarray(g) + W[:, None] * D
+    F = np.tile(np.eye(3), (len(x), 1, 1))
+    F[:, :, 1] += dt * G[:, None] * D
+    return x + dt * v1, v1, F, W
+
+
+def test_step_grid_contact_closed_form():
+    """Pins S14 (stress_update and G2P read the post-collision grid velocity):
+    reading it before grid_collisions would give v = v0 + dt g and F_E = I."""
+    x, v0, vc, (pt, nrm, mu) = grid_contact_case()
+    prm = o.params(1e-4, 0.02, *o.lame(1.4e5, 0.2), 10, 0.025, 0.0075, 0.95)
+    st = o.State(x, np.tile(v0, (len(x), 1)), 4e-4)
+    st.setup(prm.dx)
+    st.step([o.make_plane(pt, nrm, mu, velocity=vc)], prm)
+    xw, vw, Fw, W = grid_contact_expected(x, v0, vc, prm.dt, prm.dx, (0, -9.81, 0), pt[1], o.N, o.N_dash)
+    assert (W > 0.01).any() and (W == 0).any()       # contact reaches some particles, not all
+    s = np.linalg.svd(Fw, compute_uv=False)
+    assert s.max() < 1 + 0.0075 and s.min() > 1 - 0.025  # no clamp: the closed form applies
+    assert np.abs(st.field("v") - vw).max() < 1e-12
+    assert np.abs(st.field("x") - xw).max() < 1e-14
+    assert np.abs(st.field("Fe").reshape(-1, 3, 3) - Fw).max() < 1e-12
