@@ -133,6 +133,17 @@ int mpm_step(mpm_engine* e, int nsteps, mpm_timings* timings);
 int mpm_scene_download(mpm_engine* e, int scene, double* x, double* v, double* Fe, double* Fp, double* Jp,
                        double* V0);
 
+/* mpm_scene_download without device allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid allocation): the arrays are staged
+ * through a static device buffer of *max_bytes (32 MiB) that each GPU holds
+ * for the life of the process, so a request needing more staging (26 doubles
+ * per particle with every field, 3 with x alone) fails with
+ * MPM_ERR_CAPACITY.  Calls that overlap on one GPU share that buffer and
+ * race.  max_bytes may be NULL. */
+int mpm_scene_download_noalloc(mpm_engine* e, int scene, double* x, double* v, double* Fe, double* Fp, double* Jp,
+                               double* V0, size_t* max_bytes);
+
 /* Status bits OR-ed since the scene was added, and FP64 SVD fallback count. */
 int mpm_scene_status(mpm_engine* e, int scene, uint32_t* status, uint64_t* fp64_fallbacks);
 

@@ -71,6 +71,8 @@ int Engine::init(const mpm_config& cfg) {
     }
     if (cfg_.permute_period == 0) cfg_.permute_period = 16;
     HIP_TRY(hipSetDevice(cfg_.device));
+    int rc0 = scratch_setup(cfg_.device);  // the static staging buffer's address on this device
+    if (rc0) return rc0;
     HIP_TRY(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     int cus = 256;
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg_.device));
@@ -443,43 +445,84 @@ int Engine::step(int nsteps, mpm_timings* t) {
     return check_status();
 }
 
-int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0) {
+// Bytes of the staging a download of the given outputs needs (0 for none).
+int Engine::download_bytes(int sid, const double* const host[6], bool want_fp, size_t* bytes) {
     if (sid < 0 || sid >= (int)scenes_.size() || !scenes_[sid].used) {
         set_error("bad scene id");
         return MPM_ERR_ARG;
     }
-    const bool track = cfg_.track_fp != 0;
-    if (Fp && !track) {
+    if (want_fp && cfg_.track_fp == 0) {
         set_error("F_P download needs track_fp");
         return MPM_ERR_STATE;
     }
+    const size_t per[6] = {3, 3, 9, 9, 1, 1};
+    size_t total = 0;
+    for (int k = 0; k < 6; ++k)
+        if (host[k]) total += per[k] * scenes_[sid].n;
+    *bytes = total * sizeof(double);
+    return MPM_OK;
+}
+
+// The GPU writes the requested arrays into `stage` in the caller's layout
+// (doubles, original particle order; zeros for particles a failed pass
+// dropped); the host only copies them.  With fill_kernel the zeros come from
+// a fill kernel instead of hipMemsetAsync.
+int Engine::download_via(int sid, double* const host[6], double* stage, bool fill_kernel) {
     const size_t n = scenes_[sid].n;
-    // The GPU writes the requested arrays in the caller's layout (doubles,
-    // original particle order); the host only copies them.
-    double* const host[6] = {x, v, Fe, Fp, Jp, V0};
     const size_t per[6] = {3, 3, 9, 9, 1, 1};
     size_t total = 0;
     for (int k = 0; k < 6; ++k)
         if (host[k]) total += per[k] * n;
-    if (total == 0 || n == 0) return MPM_OK;
-    StagingBuffer stage(stream_);
-    HIP_TRY(hipMallocAsync((void**)&stage.p, total * sizeof(double), stream_));
-    double* const d_out = stage.p;
-    // zeros for particles the scene no longer holds (a failed pass can drop them)
-    HIP_TRY(hipMemsetAsync(d_out, 0, total * sizeof(double), stream_));
+    if (fill_kernel) {
+        launch_fill_zero(stage, total, stream_);
+        HIP_TRY(hipGetLastError());
+    } else {
+        HIP_TRY(hipMemsetAsync(stage, 0, total * sizeof(double), stream_));
+    }
     double* dev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     size_t off = 0;
     for (int k = 0; k < 6; ++k)
         if (host[k]) {
-            dev[k] = d_out + off;
+            dev[k] = stage + off;
             off += per[k] * n;
         }
     const SceneOut o{dev[0], dev[1], dev[2], dev[3], dev[4], dev[5]};
     StepArgs a = args(kSceneActive);
     launch_gather_scene(a, launch_, (uint32_t)sid, (uint32_t)n, o, stream_);
+    HIP_TRY(hipGetLastError());
     for (int k = 0; k < 6; ++k)
         if (host[k]) HIP_TRY(hipMemcpyAsync(host[k], dev[k], per[k] * n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    return MPM_OK;
+}
+
+int Engine::scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0) {
+    double* const host[6] = {x, v, Fe, Fp, Jp, V0};
+    size_t bytes = 0;
+    int rc = download_bytes(sid, host, Fp != nullptr, &bytes);
+    if (rc) return rc;
+    if (bytes == 0) return MPM_OK;
+    StagingBuffer stage(stream_);
+    HIP_TRY(hipMallocAsync((void**)&stage.p, bytes, stream_));
+    if ((rc = download_via(sid, host, stage.p, false))) return rc;
     HIP_TRY(stage.release());
+    HIP_TRY(hipStreamSynchronize(stream_));
+    return MPM_OK;
+}
+
+int Engine::scene_download_static(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0,
+                                  double* scratch, size_t scratch_bytes) {
+    double* const host[6] = {x, v, Fe, Fp, Jp, V0};
+    size_t bytes = 0;
+    int rc = download_bytes(sid, host, Fp != nullptr, &bytes);
+    if (rc) return rc;
+    if (bytes == 0) return MPM_OK;
+    if (bytes > scratch_bytes) {
+        set_error("mpm_scene_download_noalloc: the requested arrays need " + std::to_string(bytes) +
+                  " bytes of staging, more than the library's " + std::to_string(scratch_bytes) +
+                  "-byte static buffer; download fewer fields or use mpm_scene_download");
+        return MPM_ERR_CAPACITY;
+    }
+    if ((rc = download_via(sid, host, scratch, true))) return rc;
     HIP_TRY(hipStreamSynchronize(stream_));
     return MPM_OK;
 }

@@ -14,6 +14,9 @@ namespace snowmpm {
 
 extern thread_local std::string g_last_error;
 void set_error(const std::string& s);
+// Looks up, once per device, the address of the static staging buffer of
+// mpm_scene_download_noalloc (mpm_noalloc.hip); call with `device` current.
+int scratch_setup(int device);
 
 #define HIP_TRY(expr)                                                          \
     do {                                                                       \
@@ -63,6 +66,11 @@ class Engine {
     int permute_period() const { return cfg_.permute_period; }
     int serve(mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* st);
     int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
+    // The same download staged through a caller-given device buffer of
+    // scratch_bytes (no device allocation); larger requests are refused.
+    int scene_download_static(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0,
+                              double* scratch, size_t scratch_bytes);
+    int device() const { return cfg_.device; }
     int scene_status(int sid, uint32_t* status, uint64_t* fb);
     int scene_sums(int sid, double out[7]);
     int scene_steps(int sid, uint64_t* steps) const {
@@ -92,6 +100,8 @@ class Engine {
             if (sh.used) ++sh.steps;
     }
     int device_count(uint32_t* n);
+    int download_bytes(int sid, const double* const host[6], bool want_fp, size_t* bytes);
+    int download_via(int sid, double* const host[6], double* stage, bool fill_kernel);
     int upload_scene_table(int sid);
     int check_status();
     int ensure_graph(int permute);

@@ -1651,6 +1651,11 @@ __global__ __launch_bounds__(256) void k_gather_scene(StepArgs a, uint32_t sid, 
     }
 }
 
+__global__ __launch_bounds__(256) void k_fill_zero(double* p, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
 // FP64 conservation sums of one scene: sum m, sum m x (3), sum m v (3).
 __global__ __launch_bounds__(256) void k_scene_sums(StepArgs a, uint32_t sid, double* out7) {
     __shared__ double red[7][256];
@@ -1760,6 +1765,12 @@ void launch_put_scene(const StepArgs& a, const LaunchCfg& c, uint32_t first, uin
 void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, uint32_t nscene, const SceneOut& o,
                          hipStream_t s) {
     k_gather_scene<<<c.grid_p, 256, 0, s>>>(a, sid, nscene, o);
+}
+
+void launch_fill_zero(double* p, size_t n, hipStream_t s) {
+    if (n == 0) return;
+    const size_t g = (n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048;
+    k_fill_zero<<<(unsigned)g, 256, 0, s>>>(p, n);
 }
 
 void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s) {

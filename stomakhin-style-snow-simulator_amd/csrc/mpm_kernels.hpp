@@ -57,6 +57,8 @@ void launch_put_scene(const StepArgs& a, const LaunchCfg& c, uint32_t first, uin
                       const SceneIn& in, hipStream_t s);
 void launch_gather_scene(const StepArgs& a, const LaunchCfg& c, uint32_t sid, uint32_t nscene, const SceneOut& o,
                          hipStream_t s);
+// Sets n doubles at p to zero (a kernel, so it can stand in captured work).
+void launch_fill_zero(double* p, size_t n, hipStream_t s);
 void launch_scene_sums(const StepArgs& a, uint32_t sid, double* out, int nblocks, hipStream_t s);
 
 }  // namespace snowmpm

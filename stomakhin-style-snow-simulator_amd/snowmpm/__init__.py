@@ -101,6 +101,8 @@ _SIGS = {
     "mpm_setup": (C.c_int, [C.c_void_p]),
     "mpm_step": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(StepTimings)]),
     "mpm_scene_download": (C.c_int, [C.c_void_p, C.c_int, _pd, _pd, _pd, _pd, _pd, _pd]),
+    "mpm_scene_download_noalloc": (C.c_int, [C.c_void_p, C.c_int, _pd, _pd, _pd, _pd, _pd, _pd,
+                                             C.POINTER(C.c_size_t)]),
     "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
@@ -402,10 +404,7 @@ class Engine:
         """Like step() but returns the status code instead of raising."""
         return lib.mpm_step(self.h, int(nsteps), None)
 
-    def download(self, sid, fields=None):
-        """The scene's state in its original particle order: x, v (n x 3), Fe
-        (and Fp with track_fp) (n x 3 x 3), Jp, V0 (n); fields limits it to
-        the names given (only those are gathered and copied)."""
+    def _download(self, sid, fields, noalloc):
         n = self.count(sid)
         want = set(fields) if fields is not None else {"x", "v", "Fe", "Jp", "V0", "Fp"}
         x = np.zeros((n, 3)) if "x" in want else None
@@ -414,14 +413,28 @@ class Engine:
         Jp = np.zeros(n) if "Jp" in want else None
         V0 = np.zeros(n) if "V0" in want else None
         Fp = np.zeros((n, 9)) if self.track_fp and "Fp" in want else None
-        _check(lib.mpm_scene_download(self.h, int(sid), _dptr(x), _dptr(v), _dptr(Fe), _dptr(Fp), _dptr(Jp),
-                                      _dptr(V0)))
+        ptrs = (_dptr(x), _dptr(v), _dptr(Fe), _dptr(Fp), _dptr(Jp), _dptr(V0))
+        if noalloc:
+            _check(lib.mpm_scene_download_noalloc(self.h, int(sid), *ptrs, None))
+        else:
+            _check(lib.mpm_scene_download(self.h, int(sid), *ptrs))
         out = {k: a for k, a in (("x", x), ("v", v), ("Jp", Jp), ("V0", V0)) if a is not None}
         if Fe is not None:
             out["Fe"] = Fe.reshape(n, 3, 3)
         if Fp is not None:
             out["Fp"] = Fp.reshape(n, 3, 3)
         return out
+
+    def download(self, sid, fields=None):
+        """The scene's state in its original particle order: x, v (n x 3), Fe
+        (and Fp with track_fp) (n x 3 x 3), Jp, V0 (n); fields limits it to
+        the names given (only those are gathered and copied)."""
+        return self._download(sid, fields, False)
+
+    def download_noalloc(self, sid, fields=None):
+        """download() staged through the library's static device buffer
+        (mpm_scene_download_noalloc: no device allocation, 32 MiB limit)."""
+        return self._download(sid, fields, True)
 
     def status(self, sid):
         st = C.c_uint32()
