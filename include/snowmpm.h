@@ -102,6 +102,9 @@ typedef struct {
                                  particles at their index); 1 re-sorts every step.  Fast-moving
                                  scenes run ~3% faster at 4, scenes at rest ~3% slower */
     double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
+    int p2g_chunk;            /* particles P2G stages per pass: 16 or 32; 0 = 32 */
+    int g2p_threads;          /* G2P workgroup size: 64, 128 or 256; 0 = 128.
+                                 Results depend on neither; mpm_autotune chooses both */
 } mpm_config;
 
 /* ---- engine lifetime ---- */
@@ -160,12 +163,17 @@ int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
  * Status is checked once at the end. */
 int mpm_step_graph(mpm_engine* e, int nsteps);
 
-/* Chooses the persistent grid sizes of the step kernels, then the re-sort
- * period among 16, 8, 4 and 2 (16 steps per candidate), by timing real steps
- * (results depend on neither, so tuning steps are ordinary steps).
- * steps_per_config >= 1; chosen (4 ints, may be NULL) receives
+/* Chooses the tile sizes (P2G staging chunk 16/32, G2P workgroup size
+ * 64/128/256), then the persistent grid sizes of the step kernels, then the
+ * re-sort period among 16, 8, 4 and 2 (16 steps per candidate), by timing
+ * real steps (results depend on none of them, so tuning steps are ordinary
+ * steps).  steps_per_config >= 1; chosen (4 ints, may be NULL) receives
  * {particle grid, wave-per-block grid, workgroup-per-block grid, k_blocks grid}. */
 int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]);
+
+/* The engine's current tile sizes (mpm_config's, or the ones mpm_autotune
+ * chose).  No counterpart in the reference. */
+int mpm_tile_sizes(mpm_engine* e, int* p2g_chunk, int* g2p_threads);
 
 /* The engine's current re-sort period (mpm_config.permute_period, or the one
  * mpm_autotune chose).  No counterpart in the reference. */

@@ -64,6 +64,10 @@ class Engine {
     int step_graph(int nsteps);
     int autotune(int steps_per_config, int chosen[4]);
     int permute_period() const { return cfg_.permute_period; }
+    void tile_sizes(int* chunk, int* threads) const {
+        *chunk = launch_.p2g_chunk;
+        *threads = launch_.g2p_threads;
+    }
     int serve(mpm_request* reqs, int nreq, int mode, int max_chunk, mpm_serve_stats* st);
     int scene_download(int sid, double* x, double* v, double* Fe, double* Fp, double* Jp, double* V0);
     // The same download staged through a caller-given device buffer of

@@ -523,23 +523,35 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // ch = (l>>2)&3, and holds the x nodes r = 0..3 of node (., k, l&3),
 // channel (l>>2)&3.
 //
-// A wave stages its block's particles (sorted: cell, then particle id) 32 at
-// a time (kP2GChunk) into its own LDS records (per-axis B-spline weights and derivatives
-// of grid.cpp:24-58 in grid units, m, m v, S/dx): a lane fetches its x/z
-// terms with one b128; the six y/channel terms are the same for the 4 lanes
-// of a quad, which fetch them with one b64 each and share them by DPP quad
-// broadcasts.  A group past the end of a cell reads an all-zero record.  The
-// fields of the next chunk are loaded into registers while the current one is
-// accumulated.  The cells of an x row (cx = 0..3 at fixed cy, cz) are
+// A wave stages its block's particles (sorted: cell, then particle id) CHUNK
+// (16 or 32) at a time into its own LDS records (per-axis B-spline weights and
+// derivatives of grid.cpp:24-58 in grid units, m, m v, S/dx): a lane fetches
+// its x/z terms with one b128; the six y/channel terms are the same for the 4
+// lanes of a quad, which fetch them with one b64 each and share them by DPP
+// quad broadcasts.  A group past the end of a cell reads an all-zero record.
+// The fields of the next chunk are loaded into registers while the current
+// one is accumulated.  The cells of an x row (cx = 0..3 at fixed cy, cz) are
 // consecutive in sorted order; their tiles, shifted by cx, are summed in
 // registers (x nodes 0..7) and added to the wave's LDS region once per x row
 // by read-add-write.  A wave's LDS operations complete in order, so every
-// float sum has a fixed order.  The region is then written as the block's
-// partial sums.
+// float sum has a fixed order.
+// The region holds only 4 node planes Z at a time (a ring indexed by Z mod 4):
+// cells come in z-major order, and a row of cells at cz adds to planes
+// cz..cz+3, so once rows reach cz every plane below cz is complete; it is
+// then written out as the block's partial sums for that plane and its ring
+// slot cleared for plane Z + 4.  All 7 planes of every home block are written.
+// Summation order: the particles of a cell form groups of 4 counted from the
+// cell's first particle, and staging chunks end only at cell boundaries (or,
+// inside a cell longer than the chunk, after a whole number of groups), so no
+// group spans two chunks; the MFMAs of cells at even and at odd cx accumulate
+// into two separate register sets, added in a fixed order when the row is
+// deposited.  Every float sum is therefore the same for any chunk size.
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
-// bit 0, 3 with bit 1.  part stores piece t at node offset piece_base(t)
-// (times 7 channels), channel-major, nodes x-fastest.
+// bit 0, 3 with bit 1.  part stores a home block's sums plane by plane
+// (plane Z = 4 tz + lz of the region: 343 floats), each plane as its 4 xy
+// pieces txy = tx + 2 ty (offset 7 xy_base(txy)), channel-major, nodes
+// x-fastest.
 __host__ __device__ constexpr int piece_n(int t, int axis) { return ((t >> axis) & 1) ? 3 : 4; }
 __host__ __device__ constexpr int piece_base(int t) {
     int b = 0;
@@ -547,15 +559,20 @@ __host__ __device__ constexpr int piece_base(int t) {
     return b;
 }
 static_assert(piece_base(8) == kRegion, "pieces tile the 7^3 region");
+__host__ __device__ constexpr int xy_base(int txy) {
+    int b = 0;
+    for (int u = 0; u < txy; ++u) b += piece_n(u, 0) * piece_n(u, 1);
+    return b;
+}
+static_assert(xy_base(4) == 49, "xy pieces tile a 7x7 plane");
 
-// Region of a wave: [x half 0..1][channel 0..6][Y 0..6][Z 0..6][x 0..3] (x half
-// h holds x nodes 4h..4h+3).  The channel stride is padded from 196 to 208
-// floats so the b128 accesses of 8 consecutive lanes (4 z planes x 2
-// channels) fall into distinct banks.
-constexpr int kP2GStrZ = 4, kP2GStrY = 7 * kP2GStrZ, kP2GStrCh = 208, kP2GStrH = kGridCh * kP2GStrCh;
+// Region of a wave: [x half 0..1][channel 0..6][Y 0..6][Z mod 4][x 0..3] (x
+// half h holds x nodes 4h..4h+3).  The channel stride, 112 floats, is 16
+// banks off a multiple of 32, so the b128 accesses of 8 consecutive lanes
+// (4 z planes x 2 channels) fall into distinct banks.
+constexpr int kP2GStrZ = 4, kP2GStrY = 4 * kP2GStrZ, kP2GStrCh = 7 * kP2GStrY, kP2GStrH = kGridCh * kP2GStrCh;
 constexpr int kP2GReg = 2 * kP2GStrH;      // floats per wave region
-constexpr int kP2GChunk = 32;              // particles staged per pass (at most one per lane)
-static_assert(kP2GChunk <= 64, "one staged particle per lane");
+static_assert(kP2GStrCh % 32 == 16, "channel stride: 8 lanes on distinct banks");
 constexpr int kP2GRec = 40;                // floats per staged particle record
 // record layout (floats):
 //   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wz_a, dwz_a
@@ -564,31 +581,32 @@ constexpr int kP2GRec = 40;                // floats per staged particle record
 // Lane (quad member) n reads the pair [16+6c+2n, +1]; member 3's pair is
 // never used (it overlaps the next record, or the 2 floats of slack).
 constexpr int kP2GRecC = 16;
-constexpr int kP2GPst = (kP2GChunk + 1) * kP2GRec + 2;  // + the all-zero record + slack
-constexpr int kP2GFlushQ = (((kGridCh * kRegion + 63) / 64) + 1) & ~1;  // flush outputs per lane (even)
-static_assert(kP2GStrCh >= 7 * kP2GStrY, "channel stride holds 7 x 7 x 4 floats");
+// staging floats of a CHUNK: the records, the all-zero record, 2 floats of slack
+template <int CHUNK>
+constexpr int p2g_pst() { return (CHUNK + 1) * kP2GRec + 2; }
+constexpr int kP2GFlushQ = (kRegion + 63) / 64;  // plane outputs per lane
 
-// Flush table: output o of a home block's partial sums (piece t, channel c,
-// node (X, Y, Z) of the 7^3 region) is region index e[o].
-struct FlushTable {
-    uint16_t e[kGridCh * kRegion];
+// Plane table: element e of a plane's partial sums (order [txy][c][ly][lx],
+// see piece_n) is region index kPlaneSrc.e[e] + (Z mod 4) kP2GStrZ.
+struct PlaneSrc {
+    uint16_t e[kRegion];
 };
-constexpr FlushTable make_flush_table() {
-    FlushTable t{};
-    for (int pt = 0; pt < 8; ++pt) {
-        const int nx = piece_n(pt, 0), ny = piece_n(pt, 1), nz = piece_n(pt, 2), sz = nx * ny * nz;
+constexpr PlaneSrc make_plane_src() {
+    PlaneSrc t{};
+    for (int txy = 0; txy < 4; ++txy) {
+        const int nx = piece_n(txy, 0), ny = piece_n(txy, 1);
         for (int c = 0; c < kGridCh; ++c)
-            for (int idx = 0; idx < sz; ++idx) {
-                const int X = 4 * (pt & 1) + idx % nx, Y = 4 * ((pt >> 1) & 1) + (idx / nx) % ny,
-                          Z = 4 * (pt >> 2) + idx / (nx * ny);
-                t.e[kGridCh * piece_base(pt) + c * sz + idx] =
-                    (uint16_t)((X >> 2) * kP2GStrH + c * kP2GStrCh + Y * kP2GStrY + Z * kP2GStrZ + (X & 3));
-            }
+            for (int ly = 0; ly < ny; ++ly)
+                for (int lx = 0; lx < nx; ++lx) {
+                    const int X = 4 * (txy & 1) + lx, Y = 4 * (txy >> 1) + ly;
+                    t.e[kGridCh * xy_base(txy) + c * nx * ny + ly * nx + lx] =
+                        (uint16_t)((X >> 2) * kP2GStrH + c * kP2GStrCh + Y * kP2GStrY + (X & 3));
+                }
     }
     return t;
 }
-static_assert(kP2GStrH + 6 * kP2GStrCh + 6 * kP2GStrY + 6 * kP2GStrZ + 2 < kP2GReg, "flush reads in bounds");
-__constant__ FlushTable kFlush = make_flush_table();
+static_assert(kP2GReg < 65536, "16-bit plane table entries");
+__constant__ PlaneSrc kPlaneSrc = make_plane_src();
 
 typedef float p2g_v4f __attribute__((ext_vector_type(4)));
 
@@ -608,8 +626,8 @@ struct P2GHeader {
 };
 __device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int lane, P2GHeader& h) {
     h.key = a.g.keys[slot];
-    h.off = a.g.blkoff[slot];
-    h.nb = (int)a.g.blkcnt[slot];
+    h.off = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.g.blkoff[slot]);
+    h.nb = __builtin_amdgcn_readfirstlane((int)a.g.blkcnt[slot]);
     h.cbeg = (int)a.g.celloff[(size_t)slot * 64 + lane];
     h.cend = h.cbeg + (int)a.g.cellcnt[(size_t)slot * 64 + lane];
 }
@@ -669,50 +687,97 @@ __device__ __forceinline__ void p2g_add4(float* p, float t0, float t1, float t2,
     *reinterpret_cast<float4*>(p) = v;
 }
 
+// Length of the staging chunk that starts at sorted position c of a block of
+// nb particles whose cell l starts at cbeg (lane l) (wave-uniform): up to
+// CHUNK particles, ending at the last cell boundary that
+// fits, or after CHUNK particles (a whole number of groups) when the cell at
+// c is longer than that.  c is a cell start or such a split point.
+template <int CHUNK>
+__device__ __forceinline__ int p2g_chunk_len(int cbeg, int nb, int c) {
+    const int end = c + CHUNK;
+    if (end >= nb) return nb - c;
+    const uint64_t starts = __ballot(cbeg <= end);  // cells are in order: a prefix of the lanes
+    const int l = 63 - __clzll((long long)starts);
+    const int b = __builtin_amdgcn_readlane(cbeg, l);
+    return b > c ? b - c : CHUNK;
+}
+
+template <int CHUNK>
 __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
+    static_assert(CHUNK <= 64 && CHUNK % 4 == 0, "one staged particle per lane, whole groups");
+    constexpr int kPst = p2g_pst<CHUNK>();
     __shared__ __align__(16) float reg[kP2GReg];
-    __shared__ __align__(16) float pst[kP2GPst];
+    __shared__ __align__(16) float pst[kPst];
     const int lane = (int)threadIdx.x;
     const int kq = lane >> 4, n = lane & 15, na = n & 3, nb = n >> 2;
-    // this lane's deposit address for x row (cy, cz) = (0, 0): nodes
-    // (X, Y = cy + kq, Z = cz + na), channel nb (C_M) or 4 + nb (C_F)
-    const int dep0 = nb * kP2GStrCh + kq * kP2GStrY + na * kP2GStrZ;
+    // this lane's deposit address for x row (cy, cz) = (0, 0) without the
+    // ring slot: nodes (X, Y = cy + kq, Z = cz + na), channel nb (C_M) or
+    // 4 + nb (C_F)
+    const int dep0 = nb * kP2GStrCh + kq * kP2GStrY;
     const uint32_t nh = a.cnt->n_home;
     const uint32_t G = gridDim.x;
     uint32_t blk = blockIdx.x;
     if (blk >= nh) return;
     for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < kP2GRec + 2) pst[kP2GChunk * kP2GRec + lane] = 0.0f;
+    if (lane < kP2GRec + 2) pst[CHUNK * kP2GRec + lane] = 0.0f;
+    // this lane's plane outputs e = lane + 64 q: their region indices
+    // (ring slot 0), two per register
+    uint32_t fsrc[(kP2GFlushQ + 1) / 2];
+#pragma unroll
+    for (int q = 0; q < kP2GFlushQ; q += 2) {
+        const int e0 = lane + 64 * q, e1 = e0 + 64;
+        fsrc[q / 2] = (e0 < kRegion ? kPlaneSrc.e[e0] : 0u) | ((e1 < kRegion ? (uint32_t)kPlaneSrc.e[e1] : 0u) << 16);
+    }
 
     // Chunk pipeline over this wave's blocks blk, blk + G, ...: while chunk k
     // is accumulated, the fields of chunk k + 1 and the sorted indices of
-    // chunk k + 2 are loaded.  Headers are kept for blocks blk, blk + G, blk + 2G.
+    // chunk k + 2 are loaded.  Headers are kept for blocks blk, blk + G, blk + 2G
+    // (hdr 0, 1, 2); a chunk is (hdr, start, len).
     P2GHeader h{}, hn{}, hnn{};
     p2g_header(a, a.g.home_list[blk], lane, h);
     if (blk + G < nh) p2g_header(a, a.g.home_list[blk + G], lane, hn);
     if (blk + 2 * G < nh) p2g_header(a, a.g.home_list[blk + 2 * G], lane, hnn);
-    // This lane's particle index in chunk c of a block.  The load is issued
+    struct Pos {
+        int hdr, start, len;
+    };
+    // (every branch on p.hdr is wave-uniform and names the header it uses: a
+    // header picked by a run-time index would live in scratch memory)
+    auto exists = [&](int k) { return blk + (uint32_t)k * G < nh; };
+    auto after = [&](const Pos& p) -> Pos {  // p.hdr is 0 or 1
+        const int e = p.start + p.len;
+        if (p.hdr == 0) {
+            if (e < h.nb) return Pos{0, e, p2g_chunk_len<CHUNK>(h.cbeg, h.nb, e)};
+            return Pos{1, 0, exists(1) ? p2g_chunk_len<CHUNK>(hn.cbeg, hn.nb, 0) : 0};
+        }
+        if (exists(1) && e < hn.nb) return Pos{1, e, p2g_chunk_len<CHUNK>(hn.cbeg, hn.nb, e)};
+        return Pos{2, 0, exists(2) ? p2g_chunk_len<CHUNK>(hnn.cbeg, hnn.nb, 0) : 0};
+    };
+    // This lane's particle index in chunk p.  The load is issued
     // unconditionally (sorted position 0 when the lane has no particle there)
     // and the flag says whether it is used, so no branch forces a wait on the
     // loads in flight.
     bool ok;
-    auto chunk_index = [&](const P2GHeader& hh, bool exists, int c) -> uint32_t {
-        ok = exists && lane < kP2GChunk && lane < hh.nb - c;
-        return a.bin.sidx[ok ? hh.off + c + lane : 0u];
+    auto chunk_index = [&](const Pos& p) -> uint32_t {
+        ok = exists(p.hdr) && lane < p.len;
+        const uint32_t o0 = h.off, o1 = hn.off, o2 = hnn.off;  // read all, then select values
+        const uint32_t off = p.hdr == 0 ? o0 : (p.hdr == 1 ? o1 : o2);
+        return a.bin.sidx[ok ? off + p.start + lane : 0u];
     };
+    Pos cur{0, 0, p2g_chunk_len<CHUNK>(h.cbeg, h.nb, 0)};
+    Pos nxt = after(cur);
     P2GRaw raw{};
     {
-        const uint32_t i0 = chunk_index(h, true, 0);
+        const uint32_t i0 = chunk_index(cur);
         if (ok) p2g_load(a, i0, raw);
     }
-    uint32_t inext = kP2GChunk < h.nb ? chunk_index(h, true, kP2GChunk) : chunk_index(hn, blk + G < nh, 0);
+    uint32_t inext = chunk_index(nxt);
     bool inext_ok = ok;
 
     // operands of the group at chunk position p of a cell ending at hi (the
     // all-zero record for the lanes past the end)
     const char* lane_x = reinterpret_cast<const char*>(pst + kq * kP2GRec + 4 * na);
     const char* lane_c = reinterpret_cast<const char*>(pst + kq * kP2GRec + kP2GRecC + 6 * nb + 2 * na);
-    const int zero_b = (kP2GChunk - kq) * kP2GRec * 4;  // bytes from record kq to the zero record
+    const int zero_b = (CHUNK - kq) * kP2GRec * 4;  // bytes from record kq to the zero record
     auto fetch = [&](int p, int hi, float4& x, float2& c) {
         const int b = kq < hi - p ? p * (kP2GRec * 4) : zero_b;  // hi - p and p * 160 are wave-uniform
         x = *reinterpret_cast<const float4*>(lane_x + b);  // wx_a dwx_a wz_d dwz_d
@@ -724,35 +789,55 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         int bx, by, bz;
         unpack_key(h.key, sid, bx, by, bz);
         const float inv_dx = a.scenes[sid].inv_dx;
-        // x-row accumulators: C_M and C_F of x nodes 0..7 of the current x row
-        float am[8], af[8];
+        const bool write = blk < a.g.max_home;
+        if (!write && lane == 0) flag(a.st, sid, kStatusOverflow);
+        float* out = a.g.part + (size_t)(write ? blk : 0u) * kGridCh * kRegion;
+        // Node plane Z is complete: write it as the block's partial sums for
+        // that plane (343 consecutive floats) and clear its ring slot.
+        int zf = 0;  // next plane to write
+        auto flush_plane = [&](int Z) {
+            const int slot = (Z & 3) * kP2GStrZ;
+            float* po = out + Z * (kGridCh * 49);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) am[e] = af[e] = 0.0f;
+            for (int q = 0; q < kP2GFlushQ; ++q) {
+                const int e = lane + 64 * q;
+                if (e < kRegion) {
+                    float* src = reg + (int)((fsrc[q / 2] >> (16 * (q & 1))) & 0xFFFFu) + slot;
+                    const float v = *src;
+                    *src = 0.0f;
+                    if (write) po[e] = v;
+                }
+            }
+        };
+        // x-row accumulators, C_M and C_F: cells at even cx add to x nodes
+        // 0..5 (aE), cells at odd cx to x nodes 1..6 (aO, index x - 1)
+        float amE[6], afE[6], amO[6], afO[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) amE[e] = afE[e] = amO[e] = afO[e] = 0.0f;
         int row = 0;  // current x row (cy + 4 cz)
         int l = 0;    // next cell to visit
         auto deposit_row = [&]() {
-            float* d = reg + dep0 + (row & 3) * kP2GStrY + (row >> 2) * kP2GStrZ;
-            p2g_add4(d, am[0], am[1], am[2], am[3]);
-            p2g_add4(d + kP2GStrH, am[4], am[5], am[6], am[7]);
+            const int cy = row & 3, cz = row >> 2;
+            for (; zf < cz; ++zf) flush_plane(zf);  // rows have reached cz: planes below it are complete
+            __asm__ __volatile__("" ::: "memory");
+            float* d = reg + dep0 + cy * kP2GStrY + ((cz + na) & 3) * kP2GStrZ;
+            p2g_add4(d, amE[0], amE[1] + amO[0], amE[2] + amO[1], amE[3] + amO[2]);
+            p2g_add4(d + kP2GStrH, amE[4] + amO[3], amE[5] + amO[4], amO[5], 0.0f);
             if (nb < 3) {
-                p2g_add4(d + 4 * kP2GStrCh, af[0], af[1], af[2], af[3]);
-                p2g_add4(d + 4 * kP2GStrCh + kP2GStrH, af[4], af[5], af[6], af[7]);
+                p2g_add4(d + 4 * kP2GStrCh, afE[0], afE[1] + afO[0], afE[2] + afO[1], afE[3] + afO[2]);
+                p2g_add4(d + 4 * kP2GStrCh + kP2GStrH, afE[4] + afO[3], afE[5] + afO[4], afO[5], 0.0f);
             }
             __asm__ __volatile__("" ::: "memory");  // keep the rows' read-add-writes in program order
 #pragma unroll
-            for (int e = 0; e < 8; ++e) am[e] = af[e] = 0.0f;
+            for (int e = 0; e < 6; ++e) amE[e] = afE[e] = amO[e] = afO[e] = 0.0f;
         };
-        for (int c0 = 0; c0 < h.nb; c0 += kP2GChunk) {
-            const int cn = min(kP2GChunk, h.nb - c0);
+        for (;;) {  // the chunks of this block
+            const int c0 = cur.start, cn = cur.len;
             if (lane < cn) p2g_stage(raw, inv_dx, pst + lane * kP2GRec);
             // fields of chunk k + 1, then the indices of chunk k + 2
             if (inext_ok) p2g_load(a, inext, raw);
-            if (c0 + kP2GChunk < h.nb)  // chunk k + 1 is (blk, c0 + chunk)
-                inext = c0 + 2 * kP2GChunk < h.nb ? chunk_index(h, true, c0 + 2 * kP2GChunk)
-                                                  : chunk_index(hn, blk + G < nh, 0);
-            else                        // chunk k + 1 is (blk + G, 0)
-                inext = kP2GChunk < hn.nb ? chunk_index(hn, blk + G < nh, kP2GChunk)
-                                          : chunk_index(hnn, blk + 2 * G < nh, 0);
+            const Pos nn = after(nxt);
+            inext = chunk_index(nn);
             inext_ok = ok;
 
             // cells l.. that reach into this chunk, in sorted order.  The
@@ -781,13 +866,17 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                         fetch(lo, hi, x0, q0);
                         fetch(lo + 4, hi, x1, q1);
                         __builtin_amdgcn_sched_barrier(0);  // both groups' reads in flight before the MFMAs
-                        // even cx: the MFMAs accumulate straight into x nodes cx..cx+3
-                        // of the row (an even-aligned register quad, as MFMA operands
-                        // must be); odd cx: a fresh tile, added afterwards
-                        p2g_v4f cm = {0.0f, 0.0f, 0.0f, 0.0f}, cf = {0.0f, 0.0f, 0.0f, 0.0f};
-                        if ((cx & 1) == 0) {
-                            cm = p2g_v4f{am[cx], am[cx + 1], am[cx + 2], am[cx + 3]};
-                            cf = p2g_v4f{af[cx], af[cx + 1], af[cx + 2], af[cx + 3]};
+                        // the MFMAs accumulate straight into the row registers of
+                        // x nodes cx..cx+3 (an even-aligned quad of aE or aO, as
+                        // MFMA operands must be)
+                        const int o = cx & ~1;
+                        p2g_v4f cm, cf;
+                        if (cx & 1) {
+                            cm = p2g_v4f{amO[o], amO[o + 1], amO[o + 2], amO[o + 3]};
+                            cf = p2g_v4f{afO[o], afO[o + 1], afO[o + 2], afO[o + 3]};
+                        } else {
+                            cm = p2g_v4f{amE[o], amE[o + 1], amE[o + 2], amE[o + 3]};
+                            cf = p2g_v4f{afE[o], afE[o + 1], afE[o + 2], afE[o + 3]};
                         }
                         auto group = [&](const float4& rx, const float2& rc) {
                             const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
@@ -808,12 +897,12 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                             fetch(p, hi, x0, q0);
                             group(x0, q0);
                         }
-                        if ((cx & 1) == 0) {
+                        if (cx & 1) {
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r + cx] = cm[r], af[r + cx] = cf[r];
+                            for (int r = 0; r < 4; ++r) amO[o + r] = cm[r], afO[o + r] = cf[r];
                         } else {
 #pragma unroll
-                            for (int r = 0; r < 4; ++r) am[r + cx] += cm[r], af[r + cx] += cf[r];
+                            for (int r = 0; r < 4; ++r) amE[o + r] = cm[r], afE[o + r] = cf[r];
                         }
                     }
                     if (ce > cn)  // continues in the next chunk
@@ -822,40 +911,21 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                         ++l;
                 }
             }
+            // advance the chunk pipeline: chunk k + 1 becomes current
+            const bool last = nxt.hdr != cur.hdr;  // chunk k + 1 belongs to the next block
+            cur = nxt;
+            nxt = nn;
+            if (last) break;
         }
         deposit_row();
-
-        // Partial sums of the 7x7x7 node region, stored as the eight pieces
-        // that fall into the blocks H + t (layout at piece_n) with coalesced
-        // writes (table kFlush: region index of each output), clearing the
-        // region for the next block.
-        const uint32_t slot3 = blk + 3 * G < nh ? a.g.home_list[blk + 3 * G] : 0u;
-        if (blk < a.g.max_home) {
-            // This lane's flush outputs o = lane + 64 q and their region
-            // indices, two 16-bit indices per register.
-            uint32_t ftab[kP2GFlushQ / 2];
-#pragma unroll
-            for (int q = 0; q < kP2GFlushQ; q += 2) {
-                const int o0 = lane + 64 * q, o1 = o0 + 64;
-                const uint32_t e0 = o0 < kGridCh * kRegion ? kFlush.e[o0] : 0u;
-                const uint32_t e1 = o1 < kGridCh * kRegion ? kFlush.e[o1] : 0u;
-                ftab[q / 2] = e0 | (e1 << 16);
-            }
-            float* out = a.g.part + (size_t)blk * kGridCh * kRegion;
-#pragma unroll
-            for (int q = 0; q < kP2GFlushQ; ++q) {
-                const int o = lane + 64 * q;
-                if (o < kGridCh * kRegion) out[o] = reg[(ftab[q / 2] >> (16 * (q & 1))) & 0xFFFFu];
-            }
-        } else if (lane == 0) {
-            flag(a.st, sid, kStatusOverflow);
-        }
-        __asm__ __volatile__("" ::: "memory");  // region reads before the clearing writes
-        if (blk + G < nh)  // another block follows in this wave
-            for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (; zf < 7; ++zf) flush_plane(zf);  // the planes of the last rows
+        __asm__ __volatile__("" ::: "memory");
+        // shift the headers and the chunk positions to the next block
         h = hn;
         hn = hnn;
-        if (blk + 3 * G < nh) p2g_header(a, slot3, lane, hnn);
+        if (blk + 3 * G < nh) p2g_header(a, a.g.home_list[blk + 3 * G], lane, hnn);
+        cur.hdr -= 1;
+        nxt.hdr -= 1;
     }
 }
 
@@ -898,13 +968,15 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
         for (int c = 0; c < kGridCh; ++c) q[c] = 0.0f;
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int nx = piece_n(t, 0), ny = piece_n(t, 1), nz = piece_n(t, 2), sz = nx * ny * nz;
+            // node (lx, ly, lz) of piece t of home block hb: plane 4 tz + lz of
+            // its region, xy piece t & 3 (layout at xy_base)
+            const int nx = piece_n(t, 0), ny = piece_n(t, 1), nz = piece_n(t, 2), nxy = nx * ny;
             const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)revl, t);
             if (hb != kInvalid && lx < nx && ly < ny && lz < nz) {
-                const float* src =
-                    a.g.part + (size_t)hb * kGridCh * kRegion + kGridCh * piece_base(t) + (lz * ny + ly) * nx + lx;
+                const float* src = a.g.part + (size_t)hb * kGridCh * kRegion +
+                                   (4 * (t >> 2) + lz) * (kGridCh * 49) + kGridCh * xy_base(t & 3) + ly * nx + lx;
 #pragma unroll
-                for (int c = 0; c < kGridCh; ++c) q[c] += src[c * sz];
+                for (int c = 0; c < kGridCh; ++c) q[c] += src[c * nxy];
             }
         }
         float* out = a.g.vu + (size_t)slot * kVelCh * 64;
@@ -1203,11 +1275,10 @@ __device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t
 // lookup per thread and block suffices and the loads are 128-byte runs.
 constexpr int kG2PStage = kVelCh * 64 / 32;
 static_assert(kG2PStage * 32 == kVelCh * 64, "staging split");
-// Threads per G2P workgroup: one workgroup per home block, so a block's
-// particles take ceil(count / T) passes; 128 leaves fewer idle lanes in the
-// last pass than 256 for the 100-500-particle blocks of uneven scenes.
-constexpr int kG2PThreads = 128;
-static_assert(kG2PThreads % 32 == 0 && 8 % (kG2PThreads / 32) == 0, "neighbour blocks per 32-thread group");
+// Threads per G2P workgroup (T = 64, 128 or 256, chosen by the autotuner):
+// one workgroup per home block, so a block's particles take ceil(count / T)
+// passes; smaller T leaves fewer idle lanes in the last pass for the
+// 100-500-particle blocks of uneven scenes, larger T fewer staging rounds.
 
 __device__ __forceinline__ void g2p_stage_load(const StepArgs& a, uint32_t ts, int l32, float (&r)[kG2PStage]) {
     const float* src = a.g.vu + (size_t)ts * kVelCh * 64;
@@ -1241,8 +1312,9 @@ __device__ __forceinline__ void g2p_stage_store(float4* va, float2* ub, int tg, 
 // Four waves per SIMD: the register budget (<= 128 VGPRs) is met without
 // spills, and eight workgroups per CU hide each other's staging and particle
 // loads.
-template <bool TRACK, bool PERMUTE, bool PROJECT>
+template <int kG2PThreads, bool TRACK, bool PERMUTE, bool PROJECT>
 __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_g2p(StepArgs a) {
+    static_assert(kG2PThreads % 32 == 0 && 8 % (kG2PThreads / 32) == 0, "neighbour blocks per 32-thread group");
     __shared__ float4 va[512];  // [Z][Y][X]: v0 v1 v2 u0
     __shared__ float2 ub[512];  // [Z][Y][X]: u1 u2
     // The block's scene (parameters and colliders), copied to LDS once per
@@ -1263,12 +1335,16 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
         int bx, by, bz;
         unpack_key(a.g.keys[slot], sid, bx, by, bz);
         // The scene record is loaded into registers with the staging loads and
-        // written to LDS after the barrier (at most two words per thread).
+        // written to LDS after the barrier (kSW words per thread).
         constexpr int kSceneWords = (int)(sizeof(SceneDev) / 4);
-        static_assert(kSceneWords <= 2 * kG2PThreads, "two scene words per thread");
+        constexpr int kSW = (kSceneWords + kG2PThreads - 1) / kG2PThreads;
         const int* ssrc = reinterpret_cast<const int*>(&a.scenes[sid]);
-        const int t0 = (int)threadIdx.x, t1 = (int)threadIdx.x + kG2PThreads;
-        const int sw0 = t0 < kSceneWords ? ssrc[t0] : 0, sw1 = t1 < kSceneWords ? ssrc[t1] : 0;
+        int sw[kSW];
+#pragma unroll
+        for (int k = 0; k < kSW; ++k) {
+            const int t = (int)threadIdx.x + k * kG2PThreads;
+            sw[k] = t < kSceneWords ? ssrc[t] : 0;
+        }
         {
             float stg[8 / kGroups][kG2PStage];
 #pragma unroll
@@ -1280,8 +1356,11 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
         }
         {  // after the barrier above: no thread still reads the previous block's copy
             int* dst = reinterpret_cast<int*>(&ssc);
-            if (t0 < kSceneWords) dst[t0] = sw0;
-            if (t1 < kSceneWords) dst[t1] = sw1;
+#pragma unroll
+            for (int k = 0; k < kSW; ++k) {
+                const int t = (int)threadIdx.x + k * kG2PThreads;
+                if (t < kSceneWords) dst[t] = sw[k];
+            }
         }
         const SceneDev& sc = ssc;
         if (threadIdx.x < 64) a.g.cellcnt[(size_t)slot * 64 + threadIdx.x] = 0u;
@@ -1696,14 +1775,31 @@ inline int grid_grid(const LaunchCfg& c) { return c.grid_b > c.grid_h / 2 ? c.gr
 // G2P: a workgroup per home block as well.
 inline int g2p_grid(const LaunchCfg& c) { return c.grid_w > c.grid_h ? c.grid_w : c.grid_h; }
 
+// P2G with the staging chunk the launch configuration names (16 or 32).
+inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    if (c.p2g_chunk == 16)
+        k_p2g<16><<<p2g_grid(c), 64, 0, s>>>(a);
+    else
+        k_p2g<32><<<p2g_grid(c), 64, 0, s>>>(a);
+}
+
 // G2P with or without the position projection (a branch per particle only
-// when some scene asks for it).
+// when some scene asks for it), at the launch configuration's workgroup size.
+template <int T, bool TRACK, bool PERMUTE>
+void launch_g2p_t(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    if (a.project)
+        k_g2p<T, TRACK, PERMUTE, true><<<g2p_grid(c), T, 0, s>>>(a);
+    else
+        k_g2p<T, TRACK, PERMUTE, false><<<g2p_grid(c), T, 0, s>>>(a);
+}
 template <bool TRACK, bool PERMUTE>
 void launch_g2p(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
-    if (a.project)
-        k_g2p<TRACK, PERMUTE, true><<<g2p_grid(c), kG2PThreads, 0, s>>>(a);
+    if (c.g2p_threads == 64)
+        launch_g2p_t<64, TRACK, PERMUTE>(a, c, s);
+    else if (c.g2p_threads == 256)
+        launch_g2p_t<256, TRACK, PERMUTE>(a, c, s);
     else
-        k_g2p<TRACK, PERMUTE, false><<<g2p_grid(c), kG2PThreads, 0, s>>>(a);
+        launch_g2p_t<128, TRACK, PERMUTE>(a, c, s);
 }
 
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
@@ -1714,7 +1810,7 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[1], s);
-    k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
+    launch_p2g(a, c, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     k_grid<<<grid_grid(c), 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
@@ -1740,7 +1836,7 @@ void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
     k_scatter<<<c.grid_p, 256, 0, s>>>(a);
     k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
-    k_p2g<<<p2g_grid(c), 64, 0, s>>>(a);
+    launch_p2g(a, c, s);
     StepArgs am = a;
     am.mass_only = 1;
     k_grid<<<grid_grid(c), 256, 0, s>>>(am);

@@ -93,6 +93,33 @@ int Engine::autotune(int spc, int chosen[4]) {
     if ((rc = run(best, &best_ms))) return rc;  // warm-up
     if ((rc = run(best, &best_ms))) return rc;
     const int maxp = (int)std::min<size_t>((cfg_.particle_capacity + 255) / 256, (size_t)cus_ * 64);
+    // Tile sizes first (P2G staging chunk, G2P workgroup size), then the
+    // persistent grid sizes, each by coordinate descent.
+    {
+        const int chunks[] = {32, 16}, threads[] = {128, 64, 256};
+        for (int k = 0; k < 2; ++k) {
+            LaunchCfg c = best;
+            c.p2g_chunk = chunks[k];
+            if (std::memcmp(&c, &best, sizeof c) == 0) continue;
+            double ms = 0;
+            if ((rc = run(c, &ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            LaunchCfg c = best;
+            c.g2p_threads = threads[k];
+            if (std::memcmp(&c, &best, sizeof c) == 0) continue;
+            double ms = 0;
+            if ((rc = run(c, &ms))) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
     const int mult_w[] = {2, 4, 8, 16}, mult_p[] = {4, 8, 16, 32}, mult_b[] = {2, 4, 8, 16}, mult_k[] = {1, 2, 4};
     for (int field = 0; field < 4; ++field) {
         const int* mult = field == 0 ? mult_w : field == 1 ? mult_p : field == 2 ? mult_b : mult_k;
@@ -415,6 +442,12 @@ int mpm_step_graph(mpm_engine* e, int nsteps) { return e ? e->eng.step_graph(nst
 
 int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]) {
     return e ? e->eng.autotune(steps_per_config, chosen) : MPM_ERR_ARG;
+}
+
+int mpm_tile_sizes(mpm_engine* e, int* p2g_chunk, int* g2p_threads) {
+    if (!e || !p2g_chunk || !g2p_threads) return MPM_ERR_ARG;
+    e->eng.tile_sizes(p2g_chunk, g2p_threads);
+    return MPM_OK;
 }
 
 int mpm_permute_period(mpm_engine* e, int* period) {

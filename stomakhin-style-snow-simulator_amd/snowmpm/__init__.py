@@ -64,7 +64,8 @@ class StepTimings(C.Structure):
 
 class Config(C.Structure):
     _fields_ = [("particle_capacity", C.c_size_t), ("hash_log2", C.c_int), ("track_fp", C.c_int),
-                ("device", C.c_int), ("permute_period", C.c_int), ("kappa", C.c_double)]
+                ("device", C.c_int), ("permute_period", C.c_int), ("kappa", C.c_double),
+                ("p2g_chunk", C.c_int), ("g2p_threads", C.c_int)]
 
 
 class Request(C.Structure):
@@ -112,6 +113,7 @@ _SIGS = {
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mpm_permute_period": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "mpm_tile_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "mpm_serve": (C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int, C.c_int, C.c_int, C.POINTER(ServeStats)]),
     "mpm_make_plane": (Collider, [_pd, _pd, C.c_double]),
     "mpm_make_box": (Collider, [_pd, _pd, C.c_double]),
@@ -261,9 +263,11 @@ def seed_mesh(verts, tris, spacing, device=0):
 class Engine:
     """Device-resident MPM engine holding any number of scenes."""
 
-    def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0, permute_period=0):
+    def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0, permute_period=0, p2g_chunk=0,
+                 g2p_threads=0):
         cfg = Config(particle_capacity=int(capacity), hash_log2=int(hash_log2), track_fp=int(bool(track_fp)),
-                     device=int(device), permute_period=int(permute_period), kappa=float(kappa))
+                     device=int(device), permute_period=int(permute_period), kappa=float(kappa),
+                     p2g_chunk=int(p2g_chunk), g2p_threads=int(g2p_threads))
         h = C.c_void_p()
         _check(lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -325,7 +329,13 @@ class Engine:
         chosen = (C.c_int * 4)()
         _check(lib.mpm_autotune(self.h, int(steps_per_config), chosen))
         return dict(grid_p=chosen[0], grid_b=chosen[1], grid_w=chosen[2], grid_k=chosen[3],
-                    permute_period=self.permute_period())
+                    permute_period=self.permute_period(), **self.tile_sizes())
+
+    def tile_sizes(self):
+        """P2G staging chunk and G2P workgroup size in use."""
+        c, t = C.c_int(), C.c_int()
+        _check(lib.mpm_tile_sizes(self.h, C.byref(c), C.byref(t)))
+        return dict(p2g_chunk=c.value, g2p_threads=t.value)
 
     def permute_period(self):
         """Steps between full re-sorts of the particle buffers."""

@@ -152,3 +152,28 @@ def test_upload_download_round_trip():
     assert np.array_equal(full["Jp"], f32(det))
     assert set(part) == {"x", "Jp"}
     assert np.array_equal(part["x"], full["x"]) and np.array_equal(part["Jp"], full["Jp"])
+
+
+def test_tile_sizes_do_not_change_bits():
+    """P2G staging chunks of 16 and 32 particles and G2P workgroups of 64,
+    128 and 256 threads give bitwise the same state: on uniformly random
+    positions (cells of 0 to ~35 particles, cells longer than a chunk) and on
+    a scene of the batched distribution, through contact."""
+    rng = np.random.default_rng(12)
+    n = 3000
+    x = rng.uniform(0.0, 0.1, size=(n, 3)) + np.array([0.31, 0.047, -0.23])
+    v = rng.normal(0.0, 0.5, size=(n, 3)) + np.array([0.0, -2.0, 0.0])
+    prm = snowmpm.params(1e-4, 0.02, *snowmpm.lame(1.4e5, 0.2), 10.0, 0.025, 0.0075, 0.95)
+    rand = dict(x=x, v=v, m=np.full(n, 4e-4), params=prm, colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
+    for sc in (rand, snowmpm.generate_scene(3, seed=5, index=7, scale=0.5)):
+        out = []
+        for chunk, thr in ((32, 128), (16, 128), (16, 64), (32, 256)):
+            with snowmpm.Engine(capacity=len(sc["x"]) + 64, p2g_chunk=chunk, g2p_threads=thr) as eng:
+                sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+                assert eng.tile_sizes() == dict(p2g_chunk=chunk, g2p_threads=thr)
+                eng.step(25)
+                out.append(eng.download(sid))
+                assert eng.status(sid)[0] == 0
+        for o in out[1:]:
+            for k in ("x", "v", "Fe", "Jp"):
+                assert np.array_equal(out[0][k], o[k]), k
