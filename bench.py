@@ -37,32 +37,117 @@ ALG_BYTES = {"p2g": 84.1, "g2p": 131.5, "grid": 11.5}
 P2G_MACS_PER_PARTICLE = 832
 
 
-def cpu_baseline(seed):
+def _kernels_sha():
+    import hashlib
+    path = os.path.join(ROOT, "stomakhin-style-snow-simulator_amd", "csrc", "mpm_kernels.hip")
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+STEP_KERNELS = {"k_bin": "bin", "k_scatter": "scatter", "k_cellsort": "cellsort", "k_blocks": "blocks",
+                "k_p2g": "p2g", "k_grid": "grid", "k_end": "end"}
+
+
+def write_traffic(fetch_csv, write_csv, steps):
+    """profiles/traffic.json from two rocprofv3 counter runs of
+    `bench.py --steps K --no-cpu --no-serving --no-seeding --copy-calibration`
+    (one with --pmc FETCH_SIZE, one with --pmc WRITE_SIZE): HBM bytes per launch
+    of each step kernel, the mean over its last `steps` dispatches, each counter
+    scaled by what it reads for the 1 GiB device copy that ends the run."""
+    import csv
+
+    def load(path):
+        rows = list(csv.DictReader(open(path)))
+        out = []
+        for r in rows:
+            name = r.get("Kernel_Name") or r.get("Kernel-Name") or ""
+            val = float(r.get("Counter_Value") or r.get("Counter-Value") or 0.0)
+            did = int(r.get("Dispatch_Id") or r.get("Dispatch-Id") or 0)
+            out.append((did, name, val))
+        out.sort()
+        return out
+
+    def per_kernel(rows, calib_bytes):
+        copies = [v for _, name, v in rows if "copyBuffer" in name]
+        factor = calib_bytes / (copies[-1] * 1024.0)
+        got = {}
+        for _, name, v in rows:
+            base = name.split("(")[0].split("<")[0].split()[-1].replace("snowmpm::", "")
+            key = STEP_KERNELS.get(base)
+            if base == "k_g2p":  # k_g2p<threads, track, permute, project>
+                targs = name.split("<", 1)[1].split(">", 1)[0].split(",")
+                key = "g2p_permute" if targs[2].strip() == "true" else "g2p"
+            if key:
+                got.setdefault(key, []).append(v * 1024.0 * factor)
+        return {k: sum(v[-steps:]) / len(v[-steps:]) for k, v in got.items()}, factor
+
+    f, ff = per_kernel(load(fetch_csv), 1 << 30)
+    w, wf = per_kernel(load(write_csv), 1 << 30)
+    res = {k: int(f.get(k, 0) + w.get(k, 0)) for k in sorted(set(f) | set(w))}
+    res["kernels_sha256"] = _kernels_sha()
+    res["note"] = (f"HBM bytes per launch of each step kernel on the 8M-particle slab: FETCH_SIZE x {ff:.3f} + "
+                   f"WRITE_SIZE x {wf:.3f} (KiB, scaled by the 1 GiB device copy), mean of the last {steps} "
+                   "dispatches")
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as fo:
+        json.dump(res, fo, indent=1)
+    print(json.dumps(res))
+
+
+def _host_info():
+    """What the CPU baseline ran on: logical CPUs, the ones this process may
+    use, OMP_NUM_THREADS and the CPU model (lscpu)."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
+
+
+def cpu_baseline(sc, seed, scale=1.0, steps=3):
+    """The FP64 oracle (the reference step restated: std::unordered_map grid
+    rebuilt every step, OpenMP on the stencil build, stress update and G2P,
+    the other stages serial) on the slab: 1 warm-up + `steps` timed steps on
+    all the host threads this process may use, capped by OMP_NUM_THREADS (the
+    GPU box's CPU share).  scale < 1 times a smaller slab of the same spacing."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import snowmpm
     import mpmref  # the oracle driver, used here only as the timed CPU baseline
     import oracle as orc
 
-    scale = 0.3  # 120 x 50 x 120 = 720,000 particles of the same slab
-    sc = snowmpm.generate_scene(2, seed=seed, scale=scale)
+    if scale != 1.0:
+        sc = snowmpm.generate_scene(2, seed=seed, scale=scale)
     n = sc["x"].shape[0]
-    threads = os.cpu_count() or 1
-    threads = min(threads, 16)
+    info = _host_info()
+    threads = info["affinity"]
+    if info["OMP_NUM_THREADS"]:
+        threads = min(threads, int(info["OMP_NUM_THREADS"]))
     prm = sc["params"]
     mu0, la0 = snowmpm.lame(sc["E"], prm.nu)
+    t_setup = time.perf_counter()
     st = orc.State(sc["x"], sc["v"], sc["m"], mu0=mu0, lambda0=la0)
     st.setup(prm.dx)
+    t_setup = time.perf_counter() - t_setup
     cs = mpmref.orc_colliders(sc["colliders"])
     op = mpmref.orc_params(prm)
     st.run(cs, op, 1, nthreads=threads)  # warm-up step
-    steps = 15
     t0 = time.perf_counter()
     st.run(cs, op, steps, nthreads=threads)
     dt = time.perf_counter() - t0
-    return {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
-            "sample": f"slab scale {scale}: {n} particles x {steps} steps (after 1 warm-up), FP64 oracle "
-                      f"(std::unordered_map grid, reference serial/OpenMP split), {dt:.1f} s"}
+    del st
+    out = {"value": n * steps / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+           "sample": f"slab {n} particles{'' if scale == 1.0 else f' (scale {scale})'}: 1 warm-up + {steps} timed "
+                     f"steps, {dt:.1f} s (setup {t_setup:.1f} s); FP64 oracle, std::unordered_map grid rebuilt "
+                     f"each step, reference serial/OpenMP split; a 500-step run would take "
+                     f"{500 * dt / steps / 3600:.2f} h (extrapolated)"}
+    out.update(info)
+    return out
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -258,14 +343,21 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-scale", type=float, default=1.0,
+                    help="CPU baseline on a slab of this scale (1 = the full 8M slab, ~32 GB of host memory)")
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the batched / served measurements and the slab-contact run")
     ap.add_argument("--no-seeding", action="store_true", help="skip the mesh-seeding measurement")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched form (256 scenes), --steps graph-replayed steps each (for profiling)")
+    ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="write profiles/traffic.json from two rocprofv3 counter CSVs (no GPU work)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
+    if args.traffic_from:
+        write_traffic(args.traffic_from[0], args.traffic_from[1], args.steps)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the engine runs on a single MI355X and has no multi-GPU path; "
               "scenes are independent, so more GPUs would each run their own share of the scenes",
@@ -307,11 +399,17 @@ def main():
     per_kernel = {k: kt[k] for k in ("p2g", "g2p", "grid")}
     dom = max(("p2g", "g2p"), key=lambda k: per_kernel[k])
     achieved = ALG_BYTES[dom] * n / (per_kernel[dom] * 1e-3) / 1e9
-    traffic = None
+    traffic, traffic_note = None, None
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get(dom)
+            tj = json.load(f)
+        if tj.get("kernels_sha256") == _kernels_sha():
+            traffic = tj.get(dom)
+            traffic_note = "rocprofv3 FETCH_SIZE/WRITE_SIZE of these kernels (profiles/traffic.json)"
+        else:
+            traffic_note = ("not measured for this kernel source (profiles/traffic.json was measured for "
+                            f"mpm_kernels.hip sha256 {str(tj.get('kernels_sha256'))[:12]})")
     # both particle kernels against HBM; P2G also against the FP32 matrix peak
     kernel_roofline = {}
     for k in ("p2g", "g2p"):
@@ -337,7 +435,7 @@ def main():
                                "per-particle E~U[1e5,2e5], box collider, dt=5e-5, FP32 with FP64 SVD fallback",
                    "particles": n, "layout": "FP32 SoA, J_P (no full F_P)"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
                      "alg_bytes_per_particle": ALG_BYTES[dom]},
         "kernel_ms": {k: round(v, 4) for k, v in kt.items() if k != "steps"},
         "kernel_roofline": kernel_roofline,
@@ -359,7 +457,7 @@ def main():
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.seed)
+        out["cpu_baseline"] = cpu_baseline(sc, args.seed, scale=args.cpu_scale)
     print(json.dumps(out))
 
 

@@ -102,7 +102,7 @@ typedef struct {
                                  particles at their index); 1 re-sorts every step.  Fast-moving
                                  scenes run ~3% faster at 4, scenes at rest ~3% slower */
     double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
-    int p2g_chunk;            /* particles P2G stages per pass: 16 or 32; 0 = 32 */
+    int p2g_chunk;            /* particles P2G stages per pass: 16, 32 or 64; 0 = 32 */
     int g2p_threads;          /* G2P workgroup size: 64, 128 or 256; 0 = 128.
                                  Results depend on neither; mpm_autotune chooses both */
 } mpm_config;
@@ -163,7 +163,7 @@ int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
  * Status is checked once at the end. */
 int mpm_step_graph(mpm_engine* e, int nsteps);
 
-/* Chooses the tile sizes (P2G staging chunk 16/32, G2P workgroup size
+/* Chooses the tile sizes (P2G staging chunk 16/32/64, G2P workgroup size
  * 64/128/256), then the persistent grid sizes of the step kernels, then the
  * re-sort period among 16, 8, 4 and 2 (16 steps per candidate), by timing
  * real steps (results depend on none of them, so tuning steps are ordinary

@@ -70,9 +70,9 @@ int Engine::init(const mpm_config& cfg) {
         return MPM_ERR_ARG;
     }
     if (cfg_.permute_period == 0) cfg_.permute_period = 16;
-    if ((cfg_.p2g_chunk != 0 && cfg_.p2g_chunk != 16 && cfg_.p2g_chunk != 32) ||
+    if ((cfg_.p2g_chunk != 0 && cfg_.p2g_chunk != 16 && cfg_.p2g_chunk != 32 && cfg_.p2g_chunk != 64) ||
         (cfg_.g2p_threads != 0 && cfg_.g2p_threads != 64 && cfg_.g2p_threads != 128 && cfg_.g2p_threads != 256)) {
-        set_error("p2g_chunk must be 0, 16 or 32 and g2p_threads 0, 64, 128 or 256");
+        set_error("p2g_chunk must be 0, 16, 32 or 64 and g2p_threads 0, 64, 128 or 256");
         return MPM_ERR_ARG;
     }
     HIP_TRY(hipSetDevice(cfg_.device));

@@ -1775,10 +1775,12 @@ inline int grid_grid(const LaunchCfg& c) { return c.grid_b > c.grid_h / 2 ? c.gr
 // G2P: a workgroup per home block as well.
 inline int g2p_grid(const LaunchCfg& c) { return c.grid_w > c.grid_h ? c.grid_w : c.grid_h; }
 
-// P2G with the staging chunk the launch configuration names (16 or 32).
+// P2G with the staging chunk the launch configuration names (16, 32 or 64).
 inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     if (c.p2g_chunk == 16)
         k_p2g<16><<<p2g_grid(c), 64, 0, s>>>(a);
+    else if (c.p2g_chunk == 64)
+        k_p2g<64><<<p2g_grid(c), 64, 0, s>>>(a);
     else
         k_p2g<32><<<p2g_grid(c), 64, 0, s>>>(a);
 }
