@@ -33,7 +33,12 @@ class Params(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("dt", C.c_double), ("dx", C.c_double),
                 ("mu0", C.c_double), ("lambda0", C.c_double), ("hardening", C.c_double),
                 ("max_compression", C.c_double), ("max_stretch", C.c_double),
-                ("alpha", C.c_double)]
+                ("alpha", C.c_double), ("beta", C.c_double), ("cr_tolerance", C.c_double),
+                ("cr_max_iterations", C.c_int)]
+
+
+class CRStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("nodes", C.c_int), ("residual", C.c_double * 257)]
 
 
 class Timings(C.Structure):
@@ -78,6 +83,10 @@ def _load():
         "orc_particle_collisions": (None, [P, C.POINTER(Collider), i]),
         "orc_position_update": (None, [P, d]),
         "orc_position_projection": (None, [P, C.POINTER(Collider), i]),
+        "orc_constitutive_differential": (None, [P, pd, d, d, d, pd]),
+        "orc_krylov_nodes": (C.c_size_t, [G, C.POINTER(C.c_int)]),
+        "orc_apply_A": (None, [G, P, C.c_size_t, C.POINTER(Params), pd, pd]),
+        "orc_last_cr_stats": (None, [C.POINTER(CRStats)]),
         "orc_setup": (None, [P, C.c_size_t, d]),
         "orc_step": (None, [P, C.c_size_t, C.POINTER(Collider), i, C.POINTER(Params),
                             C.POINTER(Timings), i, G]),
@@ -213,13 +222,29 @@ class Grid:
         return dict(idx=idx, mass=mass, momentum=mom, velocity=vel, force=f)
 
 
-def params(dt, dx, mu0, lambda0, hardening, max_compression, max_stretch, alpha, gravity=(0, -9.81, 0)):
+def params(dt, dx, mu0, lambda0, hardening, max_compression, max_stretch, alpha, gravity=(0, -9.81, 0),
+           beta=0.0, cr_tolerance=1e-10, cr_max_iterations=200):
+    """step() scalars (grid.hpp:97); beta > 0 adds the semi-implicit solve."""
     p = Params()
     p.gravity[:] = list(map(float, gravity))
     p.dt, p.dx, p.mu0, p.lambda0 = float(dt), float(dx), float(mu0), float(lambda0)
     p.hardening, p.max_compression, p.max_stretch, p.alpha = (
         float(hardening), float(max_compression), float(max_stretch), float(alpha))
+    p.beta, p.cr_tolerance, p.cr_max_iterations = float(beta), float(cr_tolerance), int(cr_max_iterations)
     return p
+
+
+def constitutive_differential(p, dF, mu0, lambda0, xi):
+    out = (C.c_double * 9)()
+    _, dptr = _dp(np.asarray(dF, dtype=float).reshape(9))
+    lib.orc_constitutive_differential(C.byref(p), dptr, float(mu0), float(lambda0), float(xi), out)
+    return np.array(out[:]).reshape(3, 3)
+
+
+def last_cr_stats():
+    st = CRStats()
+    lib.orc_last_cr_stats(C.byref(st))
+    return dict(iterations=st.iterations, nodes=st.nodes, residual=np.array(st.residual[:st.iterations + 1]))
 
 
 def lame(E, nu):
@@ -273,6 +298,26 @@ class State:
         arr, nc = colliders_array(colliders)
         lib.orc_step(self.arr, self.n, arr, nc, C.byref(prm), C.byref(timings) if timings else None,
                      int(nthreads), grid.h if grid is not None else None)
+
+    def krylov_nodes(self, grid):
+        n = lib.orc_krylov_nodes(grid.h, None)
+        idx = np.zeros((n, 3), dtype=np.int32)
+        lib.orc_krylov_nodes(grid.h, idx.ctypes.data_as(C.POINTER(C.c_int)))
+        return idx
+
+    def apply_A(self, grid, prm, q):
+        q = np.ascontiguousarray(q, dtype=float)
+        y = np.zeros_like(q)
+        lib.orc_apply_A(grid.h, self.arr, self.n, C.byref(prm), q.ctypes.data_as(C.POINTER(C.c_double)),
+                        y.ctypes.data_as(C.POINTER(C.c_double)))
+        return y
+
+    def copy(self):
+        c = State.__new__(State)
+        c.n = self.n
+        c.arr = (Particle * self.n)()
+        C.memmove(c.arr, self.arr, C.sizeof(self.arr))
+        return c
 
     def run(self, colliders, prm, nsteps, nthreads=0, timings=None):
         arr, nc = colliders_array(colliders)

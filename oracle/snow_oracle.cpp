@@ -225,12 +225,17 @@ struct KeyHash {
     }
 };
 
-// GridNode, grid.hpp:11-21 (fields that never affect numerics omitted).
+// GridNode, grid.hpp:11-21 (density omitted: it never affects numerics).
+// delta_v, delta_force and the collision flag serve the semi-implicit solve
+// the reference declares (grid.hpp:18-20).
 struct Node {
     double mass = 0.0;
     V3 momentum{{0, 0, 0}};
     V3 velocity{{0, 0, 0}};
     V3 force{{0, 0, 0}};
+    V3 delta_v{{0, 0, 0}};
+    V3 delta_force{{0, 0, 0}};
+    bool constrained = false;  // a collider changed this node's velocity this step
 };
 
 using Grid = std::unordered_map<Key, Node, KeyHash>;
@@ -431,11 +436,15 @@ V3 collision_helper(const V3& x, const V3& v, const orc_collider& c) {
     return v;
 }
 
-// grid_collisions, grid.cpp:233-243.
+// grid_collisions, grid.cpp:233-243.  Also records which nodes a collider
+// changed (the semi-implicit solve holds those fixed).
 void grid_collisions(Grid& g, const orc_collider* cs, int nc, double dx) {
     for (auto& kv : g) {
         V3 xp = v3(dx * kv.first.i, dx * kv.first.j, dx * kv.first.k);
+        const V3 before = kv.second.velocity;
         for (int b = 0; b < nc; ++b) kv.second.velocity = collision_helper(xp, kv.second.velocity, cs[b]);
+        const V3& after = kv.second.velocity;
+        kv.second.constrained = after[0] != before[0] || after[1] != before[1] || after[2] != before[2];
     }
 }
 
@@ -518,6 +527,187 @@ struct GridBox {
     Grid g;
 };
 
+// ---------------------------------------------------------------------------
+// Semi-implicit velocity update.  The reference declares it and never defines
+// it (grid.hpp:77 constitutive_differential, 81 index_nodes, 89
+// deformation_differential, 90 force_differential, 91 apply_A, 92
+// Conjugate_residual; GridNode::delta_v / delta_force, grid.hpp:18-20).  This
+// restates the method of Stomakhin et al. 2013 (section 4): after
+// grid_velocity and grid_collisions, solve
+//   (I + beta dt^2 M^-1 H) v = v*
+// on the nodes with mass that no collider changed, where v* is the explicit
+// grid velocity and H q = -df(q)/dt the Hessian of the elastic energy applied
+// to a grid velocity field q:
+//   dF_p = dt (sum_i q_i grad w_ip^T) F_Ep              deformation_differential
+//   dP_p = d P / dF (F_Ep) : dF_p                       constitutive_differential
+//   df_i = - sum_p V_p0 dP_p F_Ep^T grad w_ip            force_differential
+//   A q = q - beta dt M^-1 df(q)                         apply_A
+// M A = M + beta dt^2 H is symmetric, so A is self-adjoint in the mass inner
+// product <a, b> = sum_i m_i a_i . b_i, and the conjugate-residual method in
+// that product (Conjugate_residual) minimises |b - A x|_M over the Krylov
+// space: the residual never grows.  The nodes are taken in index order
+// (index_nodes), so every sum has a fixed order.  beta = 0 gives A = I and
+// leaves v* unchanged.
+
+// Corotated P(F) = 2 mu (F - R) + lambda (J - 1) J F^-T differentiated at F
+// (force_constitutive, grid.cpp:162-172, with the same hardening):
+//   dR = U W V^T, W_ij = (M_ij - M_ji) / (s_i + s_j), M = U^T dF V
+//     (R^T dR is skew and R^T dR S + S R^T dR = R^T dF - dF^T R, S = R^T F)
+//   d(J F^-T) = d cof(F) (cofactor matrix, bilinear), dJ = cof(F) : dF
+M3 cofactor(const M3& F) {
+    M3 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            C.a[i * 3 + j] = F.a[i1 * 3 + j1] * F.a[i2 * 3 + j2] - F.a[i1 * 3 + j2] * F.a[i2 * 3 + j1];
+        }
+    return C;
+}
+M3 cofactor_differential(const M3& F, const M3& dF) {
+    M3 C;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            C.a[i * 3 + j] = dF.a[i1 * 3 + j1] * F.a[i2 * 3 + j2] + F.a[i1 * 3 + j1] * dF.a[i2 * 3 + j2] -
+                             dF.a[i1 * 3 + j2] * F.a[i2 * 3 + j1] - F.a[i1 * 3 + j2] * dF.a[i2 * 3 + j1];
+        }
+    return C;
+}
+
+M3 constitutive_differential(const Particle& p, const M3& dF, double mu0, double lambda0, double xi) {
+    const double h = std::exp(xi * (1.0 - det3(p.Fp.a)));
+    const double mu = mu0 * h, lambda = lambda0 * h;
+    const M3& F = p.Fe;
+    const double J = det3(F.a);
+    M3 U, V;
+    double S[3];
+    svd3(F.a, U.a, S, V.a);
+    const M3 Mh = m3_mul(m3_transpose(U), m3_mul(dF, V));
+    M3 W{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            if (i != j) W.a[i * 3 + j] = (Mh.a[i * 3 + j] - Mh.a[j * 3 + i]) / (S[i] + S[j]);
+    const M3 dR = m3_mul(U, m3_mul(W, m3_transpose(V)));
+    const M3 C = cofactor(F), dC = cofactor_differential(F, dF);
+    double dJ = 0.0;
+    for (int i = 0; i < 9; ++i) dJ += C.a[i] * dF.a[i];
+    M3 dP;
+    for (int i = 0; i < 9; ++i)
+        dP.a[i] = 2.0 * mu * (dF.a[i] - dR.a[i]) + lambda * (dJ * C.a[i] + (J - 1.0) * dC.a[i]);
+    return dP;
+}
+
+// deformation_differential: dF_p = dt (sum_i delta_v_i grad w_ip^T) F_Ep.
+M3 deformation_differential(const Stencil& st, const Particle& p, double dt) {
+    M3 L{};
+    for (const Entry& e : st) {
+        const V3& q = e.node->delta_v;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) L.a[r * 3 + c] += q[r] * e.gradient[c];
+    }
+    for (double& x : L.a) x *= dt;
+    return m3_mul(L, p.Fe);
+}
+
+// force_differential: delta_force_i -= V_p0 dP_p F_Ep^T grad w_ip.
+void force_differential(const Stencil& st, const Particle& p, double dt, double mu0, double lambda0, double xi) {
+    const M3 dP = constitutive_differential(p, deformation_differential(st, p, dt), mu0, lambda0, xi);
+    M3 G = m3_mul(dP, m3_transpose(p.Fe));
+    for (double& x : G.a) x *= -p.V0;
+    for (const Entry& e : st) e.node->delta_force = add(e.node->delta_force, m3_apply(G, e.gradient));
+}
+
+// index_nodes: the unknowns, nodes with mass that no collider changed, in
+// index order.
+std::vector<Node*> index_nodes(Grid& g) {
+    std::vector<std::pair<Key, Node*>> v;
+    for (auto& kv : g)
+        if (kv.second.mass > 0.0 && !kv.second.constrained) v.push_back({kv.first, &kv.second});
+    std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) {
+        if (a.first.i != b.first.i) return a.first.i < b.first.i;
+        if (a.first.j != b.first.j) return a.first.j < b.first.j;
+        return a.first.k < b.first.k;
+    });
+    std::vector<Node*> out;
+    for (auto& e : v) out.push_back(e.second);
+    return out;
+}
+
+using KrylovVector = std::vector<V3>;
+
+double particle_mu0(const Particle& p, const orc_params& prm) { return p.mu0 >= 0 ? p.mu0 : prm.mu0; }
+double particle_la0(const Particle& p, const orc_params& prm) { return p.lambda0 >= 0 ? p.lambda0 : prm.lambda0; }
+
+// apply_A: y = q - beta dt M^-1 df(q) on the unknown nodes (q = 0 elsewhere).
+KrylovVector apply_A(Grid& g, const std::vector<Particle>& snow, const std::vector<Stencil>& st,
+                     const std::vector<Node*>& kn, const KrylovVector& q, const orc_params& prm) {
+    for (auto& kv : g) {
+        kv.second.delta_v = v3(0, 0, 0);
+        kv.second.delta_force = v3(0, 0, 0);
+    }
+    for (size_t k = 0; k < kn.size(); ++k) kn[k]->delta_v = q[k];
+    for (size_t i = 0; i < snow.size(); ++i)
+        force_differential(st[i], snow[i], prm.dt, particle_mu0(snow[i], prm), particle_la0(snow[i], prm),
+                           prm.hardening);
+    KrylovVector y(kn.size());
+    for (size_t k = 0; k < kn.size(); ++k)
+        y[k] = sub(q[k], scl(prm.beta * prm.dt / kn[k]->mass, kn[k]->delta_force));
+    return y;
+}
+
+double mdot(const std::vector<Node*>& kn, const KrylovVector& a, const KrylovVector& b) {
+    double s = 0.0;
+    for (size_t k = 0; k < kn.size(); ++k) s += kn[k]->mass * dot(a[k], b[k]);
+    return s;
+}
+
+orc_cr_stats g_cr_stats{};
+
+// Conjugate_residual: solves A v = v* for the unknown nodes' velocities.
+void conjugate_residual(Grid& g, const std::vector<Particle>& snow, const std::vector<Stencil>& st,
+                        const orc_params& prm) {
+    std::vector<Node*> kn = index_nodes(g);
+    orc_cr_stats stats{};
+    stats.nodes = (int)kn.size();
+    const size_t n = kn.size();
+    KrylovVector x(n), b(n);
+    for (size_t k = 0; k < n; ++k) x[k] = b[k] = kn[k]->velocity;
+    KrylovVector Ax = apply_A(g, snow, st, kn, x, prm);
+    KrylovVector r(n);
+    for (size_t k = 0; k < n; ++k) r[k] = sub(b[k], Ax[k]);
+    const double bb = mdot(kn, b, b), tol2 = prm.cr_tolerance * prm.cr_tolerance * bb;
+    double rr = mdot(kn, r, r);
+    stats.residual[0] = std::sqrt(rr);
+    const int maxit = std::min(prm.cr_max_iterations, 256);
+    if (rr > tol2) {
+        KrylovVector p = r, Ar = apply_A(g, snow, st, kn, r, prm), Ap = Ar;
+        double rho = mdot(kn, r, Ar);
+        for (int it = 0; it < maxit; ++it) {
+            const double den = mdot(kn, Ap, Ap);
+            if (!(den > 0.0) || rho == 0.0) break;
+            const double alpha = rho / den;
+            for (size_t k = 0; k < n; ++k) {
+                x[k] = add(x[k], scl(alpha, p[k]));
+                r[k] = sub(r[k], scl(alpha, Ap[k]));
+            }
+            rr = mdot(kn, r, r);
+            stats.iterations = it + 1;
+            stats.residual[it + 1] = std::sqrt(rr);
+            if (rr <= tol2) break;
+            Ar = apply_A(g, snow, st, kn, r, prm);
+            const double rho_new = mdot(kn, r, Ar);
+            const double bet = rho_new / rho;
+            rho = rho_new;
+            for (size_t k = 0; k < n; ++k) {
+                p[k] = add(r[k], scl(bet, p[k]));
+                Ap[k] = add(Ar[k], scl(bet, Ap[k]));
+            }
+        }
+    }
+    for (size_t k = 0; k < n; ++k) kn[k]->velocity = x[k];
+    g_cr_stats = stats;
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0, std::chrono::steady_clock::time_point t1) {
     return std::chrono::duration<double, std::milli>(t1 - t0).count();
 }
@@ -570,6 +760,8 @@ void step_impl(std::vector<Particle>& snow, std::vector<Stencil>& stencils, cons
     grid_collisions(grid, cs, nc, dx);
     t1 = clk::now();
     t->grid_collisions += ms_since(t0, t1);
+
+    if (prm.beta > 0.0) conjugate_residual(grid, snow, stencils, prm);
 
     t0 = clk::now();
 #pragma omp parallel for
@@ -769,6 +961,50 @@ void orc_position_projection(orc_particle* p, const orc_collider* c, int nc) {
 }
 
 // setup, grid.cpp:340-357.
+void orc_constitutive_differential(const orc_particle* p, const double dF[9], double mu0, double lambda0, double xi,
+                                   double dP[9]) {
+    M3 d;
+    std::memcpy(d.a, dF, sizeof d.a);
+    const M3 r = constitutive_differential(from_c(*p), d, mu0, lambda0, xi);
+    std::memcpy(dP, r.a, sizeof r.a);
+}
+
+size_t orc_krylov_nodes(const orc_grid* g, int* idx) {
+    std::vector<Key> keys;
+    for (const auto& kv : g->g)
+        if (kv.second.mass > 0.0 && !kv.second.constrained) keys.push_back(kv.first);
+    std::sort(keys.begin(), keys.end(), [](const Key& a, const Key& b) {
+        if (a.i != b.i) return a.i < b.i;
+        if (a.j != b.j) return a.j < b.j;
+        return a.k < b.k;
+    });
+    if (idx)
+        for (size_t k = 0; k < keys.size(); ++k) {
+            idx[k * 3] = keys[k].i;
+            idx[k * 3 + 1] = keys[k].j;
+            idx[k * 3 + 2] = keys[k].k;
+        }
+    return keys.size();
+}
+
+void orc_apply_A(orc_grid* g, const orc_particle* ps, size_t n, const orc_params* prm, const double* q, double* y) {
+    std::vector<Particle> snow(n);
+    std::vector<Stencil> st(n);
+    for (size_t i = 0; i < n; ++i) {
+        snow[i] = from_c(ps[i]);
+        build_stencil(snow[i], prm->dx, st[i]);
+        for (Entry& e : st[i]) e.node = &g->g[e.index];
+    }
+    const std::vector<Node*> kn = index_nodes(g->g);
+    KrylovVector qv(kn.size());
+    for (size_t k = 0; k < kn.size(); ++k) qv[k] = v3(q[k * 3], q[k * 3 + 1], q[k * 3 + 2]);
+    const KrylovVector r = apply_A(g->g, snow, st, kn, qv, *prm);
+    for (size_t k = 0; k < kn.size(); ++k)
+        for (int a = 0; a < 3; ++a) y[k * 3 + a] = r[k][a];
+}
+
+void orc_last_cr_stats(orc_cr_stats* out) { *out = g_cr_stats; }
+
 void orc_setup(orc_particle* ps, size_t n, double dx) {
     std::vector<Particle> snow(n);
     for (size_t i = 0; i < n; ++i) snow[i] = from_c(ps[i]);

@@ -66,7 +66,19 @@ typedef struct {
     double max_compression; /* theta_c */
     double max_stretch;     /* theta_s */
     double alpha;           /* FLIP ratio */
+    /* Semi-implicit velocity solve (declared, never defined, by the reference:
+     * grid.hpp:77, 81, 89-92).  beta = 0: the explicit step. */
+    double beta;            /* implicit ratio in [0, 1] */
+    double cr_tolerance;    /* stop when |r|_M <= cr_tolerance |b|_M */
+    int cr_max_iterations;
 } orc_params;
+
+/* Conjugate-residual statistics of the last step's solve. */
+typedef struct {
+    int iterations;
+    int nodes;              /* unknown nodes (mass > 0, not held by a collider) */
+    double residual[257];   /* |r|_M after 0..iterations iterations */
+} orc_cr_stats;
 
 /* StepTimings, include/snow/debug.hpp:3-15 (milliseconds, accumulated). */
 typedef struct {
@@ -122,6 +134,16 @@ void orc_particle_collisions(orc_particle* p, const orc_collider* c, int nc);
 void orc_position_update(orc_particle* p, double dt);
 /* Optional projection after position_update (colliders with flags bit 0). */
 void orc_position_projection(orc_particle* p, const orc_collider* c, int nc);
+
+/* ---- semi-implicit solve (grid.hpp:77, 81, 89-92; restated design) ----
+ * orc_constitutive_differential: dP = d(P)/dF : dF of the corotated model
+ * with hardening at particle p.  orc_apply_A: y = A q on the unknown nodes of
+ * a grid built by orc_step(..., grid_out) (node order = orc_krylov_nodes). */
+void orc_constitutive_differential(const orc_particle* p, const double dF[9], double mu0, double lambda0, double xi,
+                                   double dP[9]);
+size_t orc_krylov_nodes(const orc_grid* g, int* idx);
+void orc_apply_A(orc_grid* g, const orc_particle* ps, size_t n, const orc_params* prm, const double* q, double* y);
+void orc_last_cr_stats(orc_cr_stats* out);
 
 /* ---- whole-path entry points (grid.cpp:340-450) ---- */
 void orc_setup(orc_particle* ps, size_t n, double dx);
