@@ -56,6 +56,13 @@ typedef struct {
     double max_stretch;     /* theta_s */
     double alpha;           /* FLIP ratio */
     double nu;              /* Poisson ratio used with per-particle Young's moduli */
+    /* Semi-implicit velocity solve, declared but never defined by the
+     * reference (grid.hpp:77, 81, 89-92): beta = 0 is the explicit step;
+     * beta > 0 solves (I + beta dt^2 M^-1 H) v = v* by conjugate residuals
+     * until |r|_M <= cr_tolerance |v*|_M or cr_max_iterations (<= 256). */
+    double beta;
+    double cr_tolerance;    /* 0 = 1e-6 */
+    int cr_max_iterations;  /* 0 = 100 */
 } mpm_params;
 
 /* POD collider (replaces CollisionBody, collision.hpp:6-12).  type 0 is the
@@ -178,6 +185,11 @@ int mpm_tile_sizes(mpm_engine* e, int* p2g_chunk, int* g2p_threads);
 /* The engine's current re-sort period (mpm_config.permute_period, or the one
  * mpm_autotune chose).  No counterpart in the reference. */
 int mpm_permute_period(mpm_engine* e, int* period);
+
+/* Conjugate-residual statistics of the scene's last semi-implicit solve:
+ * iterations taken and |r|_M after 0..iterations of them (at most cap
+ * values written; residual may be NULL).  0 iterations for beta = 0. */
+int mpm_cr_stats(mpm_engine* e, int scene, int* iterations, double* residual, int cap);
 
 /* Steps scene `scene` has taken since it was added (setup not counted).
  * No counterpart in the reference, whose step() takes no scene handle. */

@@ -55,6 +55,9 @@ struct SceneDev {
     int ncoll;
     int flags;                               // kSceneActive | kSceneSetup
     int nproject;                            // colliders with the projection flag
+    float beta;                              // semi-implicit ratio (0: explicit step)
+    float cr_tol2;                           // conjugate residual: stop at |r|_M^2 <= cr_tol2 |b|_M^2
+    int cr_maxit;                            // and after at most this many iterations
     ColliderDev coll[kMaxColliders];
 };
 static_assert(sizeof(SceneDev) % 4 == 0, "scene records are copied as 32-bit words");
@@ -129,6 +132,27 @@ struct FallbackDev {
     uint32_t* pos;         // [N] sorted positions whose update is redone in FP64
     float* L;              // [N*9] their velocity gradients
 };
+
+// Semi-implicit solve state (allocated on the first step of a scene with
+// beta > 0).  Node vectors share vu's layout: [slot][channel 3][node 64].
+constexpr int kCRVecs = 5;  // X, R, P, AR, AP
+struct CRScene {            // per scene, in FP64
+    double bb, rr, rho, den, alpha, beta;
+    int active, iters;
+};
+struct ImpDev {
+    float* gm;              // [H*64] node mass of the unknowns (0: massless or held by a collider)
+    float* vec[kCRVecs];    // [H*3*64] each
+    float* G;               // [9][N] per-particle -V0 dP F^T for the force differential
+    uint32_t* ord;          // [nblocks] slots of all blocks in key order
+    uint64_t* skey;         // [nblocks] their keys
+    double* dp;             // [nblocks*2] per-block dot-product partials (key order)
+    uint32_t* seg;          // [2*kMaxScenes] per scene: first, end index into ord
+    CRScene* cr;            // [kMaxScenes]
+    double* hist;           // [kMaxScenes*kCRHist] |r|_M per iteration of the last solve
+    uint32_t* nactive;      // scenes still iterating
+};
+constexpr int kCRHist = 257;
 
 struct StatusDev {
     uint32_t* scene_status;          // [kMaxScenes]

@@ -192,7 +192,53 @@ StepArgs Engine::args(uint32_t pass_flag, int permute) const {
     a.kappa = (float)cfg_.kappa;
     a.permute = permute;
     a.project = any_project_ ? 1 : 0;
+    if (implicit_active()) a.imp = imp_;
     return a;
+}
+
+bool Engine::implicit_active() const {
+    for (size_t s = 0; s < scenes_.size(); ++s)
+        if (scenes_[s].used && h_scenes_[s].beta > 0.0f) return true;
+    return false;
+}
+
+int Engine::one_step(int permute, hipEvent_t* ev, mpm_timings* t) {
+    if (!implicit_active()) {
+        const StepArgs a = args(kSceneActive, permute);
+        launch_step(a, launch_, stream_, ev);
+        advance(a.permute);
+        return MPM_OK;
+    }
+    int rc = implicit_alloc();
+    if (rc) return rc;
+    const StepArgs a = args(kSceneActive, permute);
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    struct Events {  // destroyed on every return path
+        hipEvent_t* e;
+        ~Events() {
+            for (int k = 0; k < 3; ++k)
+                if (e[k]) (void)hipEventDestroy(e[k]);
+        }
+    } guard{e};
+    if (t)
+        for (auto& x : e) HIP_TRY(hipEventCreate(&x));
+    launch_step_head(a, launch_, stream_);
+    if (t) HIP_TRY(hipEventRecord(e[0], stream_));
+    if ((rc = implicit_solve(a))) return rc;
+    if (t) HIP_TRY(hipEventRecord(e[1], stream_));
+    launch_step_tail(a, launch_, stream_);
+    if (t) HIP_TRY(hipEventRecord(e[2], stream_));
+    HIP_TRY(hipGetLastError());
+    advance(a.permute);
+    if (t) {
+        HIP_TRY(hipEventSynchronize(e[2]));
+        float solve = 0, tail = 0;
+        HIP_TRY(hipEventElapsedTime(&solve, e[0], e[1]));
+        HIP_TRY(hipEventElapsedTime(&tail, e[1], e[2]));
+        t->grid_collisions += solve;  // the solve follows grid_collisions
+        t->g2p += tail;
+    }
+    return MPM_OK;
 }
 
 bool Engine::take_permute() {
@@ -239,6 +285,10 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     }
     if (!(prm.dx > 0) || !(prm.dt > 0)) {
         set_error("mpm_scene_add: dx and dt must be positive");
+        return MPM_ERR_ARG;
+    }
+    if (!(prm.beta >= 0.0) || prm.cr_tolerance < 0.0 || prm.cr_max_iterations < 0 || prm.cr_max_iterations > 256) {
+        set_error("mpm_scene_add: beta must be >= 0, cr_tolerance >= 0 and cr_max_iterations in [0, 256]");
         return MPM_ERR_ARG;
     }
     int sid = -1;
@@ -303,6 +353,12 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     sd.theta_c = (float)prm.max_compression;
     sd.theta_s = (float)prm.max_stretch;
     sd.alpha = (float)prm.alpha;
+    sd.beta = (float)prm.beta;
+    {
+        const double tol = prm.cr_tolerance > 0.0 ? prm.cr_tolerance : 1e-6;
+        sd.cr_tol2 = (float)(tol * tol);
+    }
+    sd.cr_maxit = prm.cr_max_iterations > 0 ? prm.cr_max_iterations : 100;
     sd.use_E = E ? 1 : 0;
     sd.ncoll = ncoll;
     sd.nproject = 0;
@@ -423,6 +479,17 @@ int Engine::step(int nsteps, mpm_timings* t) {
             HIP_TRY(hipEventCreate(&ev));
             events_.push_back(ev);
         }
+    }
+    if (implicit_active()) {
+        // host-driven solve between the halves of each step: no per-kernel
+        // event slots; the solve's time is added to timings->grid_collisions
+        for (int s = 0; s < nsteps; ++s)
+            if ((rc = one_step(take_permute() ? 1 : 0, nullptr, t))) return rc;
+        HIP_TRY(hipStreamSynchronize(stream_));
+        if (nsteps > 0)
+            for (auto& sh : scenes_) sh.pending_drop = false;
+        kernel_steps_ = 0;
+        return check_status();
     }
     for (int s = 0; s < nsteps; ++s) {
         const StepArgs a = args(kSceneActive, take_permute() ? 1 : 0);

@@ -85,6 +85,7 @@ class Engine {
         *steps = scenes_[sid].steps;
         return MPM_OK;
     }
+    int cr_stats(int sid, int* iters, double* residual, int cap);
     Footprint footprint(size_t n, const double* x, double dx) const;
     Footprint pools() const;
     void kernel_times(double ms[5], int* nsteps) const {
@@ -105,6 +106,13 @@ class Engine {
     }
     int device_count(uint32_t* n);
     int download_bytes(int sid, const double* const host[6], bool want_fp, size_t* bytes);
+    // semi-implicit solve (mpm_implicit.hip)
+    bool implicit_active() const;
+    int implicit_alloc();
+    int implicit_solve(const StepArgs& a);
+    // one step of every active scene (explicit launches, or the solve between
+    // the two halves when a scene has beta > 0); ev may be null
+    int one_step(int permute, hipEvent_t* ev, mpm_timings* t);
     int download_via(int sid, double* const host[6], double* stage, bool fill_kernel);
     int upload_scene_table(int sid);
     int check_status();
@@ -145,6 +153,11 @@ class Engine {
     hipGraphExec_t gexec_[8] = {};  // [(mov_ * 2 + con_) * 2 + permute]
     StepArgs gargs_[8]{};
     LaunchCfg glaunch_[8]{};
+    ImpDev imp_{};
+    uint64_t* imp_kin_ = nullptr;
+    uint32_t* imp_vin_ = nullptr;
+    void* imp_sort_tmp_ = nullptr;
+    size_t imp_sort_bytes_ = 0;
     int cus_ = 256;
     double kernel_ms_[5] = {0, 0, 0, 0, 0};
     int kernel_steps_ = 0;

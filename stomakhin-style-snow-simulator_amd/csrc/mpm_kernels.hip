@@ -67,21 +67,6 @@ __device__ __forceinline__ bool finite3(float a, float b, float c) {
     return isfinite(a) && isfinite(b) && isfinite(c);
 }
 
-// Base node of the particle's stencil, floor(x/dx) - 1 (grid.cpp:64-67).
-__device__ __forceinline__ void base_node(float inv_dx, float x0, float x1, float x2, int b[3], float f[3]) {
-    const float g0 = x0 * inv_dx, g1 = x1 * inv_dx, g2 = x2 * inv_dx;
-    const float c0 = floorf(g0), c1 = floorf(g1), c2 = floorf(g2);
-    f[0] = g0 - c0;
-    f[1] = g1 - c1;
-    f[2] = g2 - c2;
-    b[0] = (int)c0 - 1;
-    b[1] = (int)c1 - 1;
-    b[2] = (int)c2 - 1;
-}
-__device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1, float x2, int b[3], float f[3]) {
-    base_node(sc.inv_dx, x0, x1, x2, b, f);
-}
-
 // ---------------------------------------------------------------------------
 
 __global__ void k_set_count(Counters* c, uint32_t n) {
@@ -174,21 +159,6 @@ __device__ __forceinline__ int bin_leader32(uint32_t v, bool want, uint64_t& gro
         pending &= ~mm;
     }
     return leader;
-}
-
-// Element i of a particle array as a 32-bit byte offset from the array's
-// wave-uniform base: the access takes the base from scalar registers and needs
-// no 64-bit address arithmetic per lane.  Particle indices are below 2^30
-// (the engine's capacity limit), so the offset does not wrap.
-template <typename T>
-__device__ __forceinline__ T& at(T* p, uint32_t i) {
-    static_assert(sizeof(T) == 4, "4-byte elements");
-    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (i << 2));
-}
-// Component e of a vector field (x, v, F, Fp, S) of particle i: components
-// are p.stride elements apart, so a kernel needs only the first one's base.
-__device__ __forceinline__ float& comp(float* const* f, const PBuf& p, int e, uint32_t i) {
-    return at(f[0] + (size_t)e * p.stride, i);
 }
 
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
@@ -633,28 +603,43 @@ __device__ __forceinline__ void p2g_header(const StepArgs& a, uint32_t slot, int
 }
 
 // The fields P2G reads of one particle, loaded into registers one chunk
-// ahead of their use so the global-load latency hides under the MFMAs.
+// ahead of their use so the global-load latency hides under the MFMAs.  With
+// FORCE (the force differential of the semi-implicit solve) only x and the
+// general 3x3 matrix G = -V0 dP F^T are read.
 struct P2GRaw {
-    float x[3], v[3], m, S[kStressCh];
+    float x[3], v[3], m, S[9];
 };
+template <bool FORCE>
 __device__ __forceinline__ void p2g_load(const StepArgs& a, uint32_t i, P2GRaw& r) {
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        r.x[e] = comp(a.cur.x, a.cur, e, i);
-        r.v[e] = comp(a.cur.v, a.cur, e, i);
+    for (int e = 0; e < 3; ++e) r.x[e] = comp(a.cur.x, a.cur, e, i);
+    if (FORCE) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) r.S[e] = at(a.imp.G + (size_t)e * a.cur.stride, i);
+        return;
     }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) r.v[e] = comp(a.cur.v, a.cur, e, i);
     r.m = at(a.cur.m, i);
 #pragma unroll
     for (int e = 0; e < kStressCh; ++e) r.S[e] = comp(a.cur.S, a.cur, e, i);
 }
 
-// Stage a particle as record rec: weights of the three axes, m, m v, S / dx.
+// Stage a particle as record rec: weights of the three axes, m, m v, S / dx
+// (FORCE: zero mass and momentum, row c of G / dx in place of row c of S).
+template <bool FORCE>
 __device__ __forceinline__ void p2g_stage(const P2GRaw& r, float inv_dx, float* rec) {
     const float g0 = r.x[0] * inv_dx, g1 = r.x[1] * inv_dx, g2 = r.x[2] * inv_dx;
-    const float m = r.m;
-    const float q1 = m * r.v[0], q2 = m * r.v[1], q3 = m * r.v[2];
-    const float s00 = r.S[0] * inv_dx, s11 = r.S[1] * inv_dx, s22 = r.S[2] * inv_dx;
-    const float s01 = r.S[3] * inv_dx, s02 = r.S[4] * inv_dx, s12 = r.S[5] * inv_dx;
+    const float m = FORCE ? 0.0f : r.m;
+    const float q1 = FORCE ? 0.0f : m * r.v[0], q2 = FORCE ? 0.0f : m * r.v[1], q3 = FORCE ? 0.0f : m * r.v[2];
+    float S[3][3];
+    if (FORCE) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) S[e / 3][e % 3] = r.S[e] * inv_dx;
+    } else {
+        S[0][0] = r.S[0] * inv_dx, S[1][1] = r.S[1] * inv_dx, S[2][2] = r.S[2] * inv_dx;
+        S[0][1] = S[1][0] = r.S[3] * inv_dx, S[0][2] = S[2][0] = r.S[4] * inv_dx, S[1][2] = S[2][1] = r.S[5] * inv_dx;
+    }
     float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
     bspline(g0 - floorf(g0), wx, dwx);
     bspline(g1 - floorf(g1), wy, dwy);
@@ -664,14 +649,14 @@ __device__ __forceinline__ void p2g_stage(const P2GRaw& r, float inv_dx, float* 
     for (int e = 0; e < 4; ++e) r4[e] = make_float4(wx[e], dwx[e], wz[e], dwz[e]);
     float2* r2 = reinterpret_cast<float2*>(rec + kP2GRecC);
     r2[0] = make_float2(wy[0], dwy[0]);
-    r2[1] = make_float2(m, s00);
-    r2[2] = make_float2(s01, s02);
+    r2[1] = make_float2(m, S[0][0]);
+    r2[2] = make_float2(S[0][1], S[0][2]);
     r2[3] = make_float2(wy[1], dwy[1]);
-    r2[4] = make_float2(q1, s01);
-    r2[5] = make_float2(s11, s12);
+    r2[4] = make_float2(q1, S[1][0]);
+    r2[5] = make_float2(S[1][1], S[1][2]);
     r2[6] = make_float2(wy[2], dwy[2]);
-    r2[7] = make_float2(q2, s02);
-    r2[8] = make_float2(s12, s22);
+    r2[7] = make_float2(q2, S[2][0]);
+    r2[8] = make_float2(S[2][1], S[2][2]);
     r2[9] = make_float2(wy[3], dwy[3]);
     r2[10] = make_float2(q3, 0.0f);
     r2[11] = make_float2(0.0f, 0.0f);
@@ -702,7 +687,7 @@ __device__ __forceinline__ int p2g_chunk_len(int cbeg, int nb, int c) {
     return b > c ? b - c : CHUNK;
 }
 
-template <int CHUNK>
+template <int CHUNK, bool FORCE>
 __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     static_assert(CHUNK <= 64 && CHUNK % 4 == 0, "one staged particle per lane, whole groups");
     constexpr int kPst = p2g_pst<CHUNK>();
@@ -768,7 +753,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     P2GRaw raw{};
     {
         const uint32_t i0 = chunk_index(cur);
-        if (ok) p2g_load(a, i0, raw);
+        if (ok) p2g_load<FORCE>(a, i0, raw);
     }
     uint32_t inext = chunk_index(nxt);
     bool inext_ok = ok;
@@ -833,9 +818,9 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         };
         for (;;) {  // the chunks of this block
             const int c0 = cur.start, cn = cur.len;
-            if (lane < cn) p2g_stage(raw, inv_dx, pst + lane * kP2GRec);
+            if (lane < cn) p2g_stage<FORCE>(raw, inv_dx, pst + lane * kP2GRec);
             // fields of chunk k + 1, then the indices of chunk k + 2
-            if (inext_ok) p2g_load(a, inext, raw);
+            if (inext_ok) p2g_load<FORCE>(a, inext, raw);
             const Pos nn = after(nxt);
             inext = chunk_index(nn);
             inext_ok = ok;
@@ -883,7 +868,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                             const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
                             const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
                             const float wxy = rx.x * wy;  // wx_a wy_b
-                            cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
+                            if (!FORCE) cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
                             cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
                             cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
                             cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
@@ -998,7 +983,14 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
         }
         const double dx = sc.dx;
         const double xn0 = dx * (double)(4 * bx + lx), xn1 = dx * (double)(4 * by + ly), xn2 = dx * (double)(4 * bz + lz);
+        const double vs0 = v[0], vs1 = v[1], vs2 = v[2];
         collide<double>(sc, xn0, xn1, xn2, v[0], v[1], v[2]);
+        if (a.imp.gm) {
+            // semi-implicit solve: the unknowns are the nodes with mass of
+            // scenes with beta > 0 that no collider changed
+            const bool held = v[0] != vs0 || v[1] != vs1 || v[2] != vs2;
+            a.imp.gm[(size_t)slot * 64 + lane] = (q[0] > 0.0f && !held && sc.beta > 0.0f) ? q[0] : 0.0f;
+        }
         const double al = sc.alpha;
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
@@ -1258,16 +1250,6 @@ __device__ __forceinline__ void elastic_stress(const Sym& E, float J, float mu, 
     S[5] = nv * (two_mu * f.a12);
 }
 
-__device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
-    if (sc.use_E) {
-        const float E = at(p.E, i);
-        mu0 = E * sc.mu_per_E;
-        la0 = E * sc.la_per_E;
-    } else {
-        mu0 = sc.mu0;
-        la0 = sc.lambda0;
-    }
-}
 
 // ---------------------------------------------------------------------------
 // G2P grid staging: neighbour block t of a home block (t = 0..7) is staged by
@@ -1778,11 +1760,14 @@ inline int g2p_grid(const LaunchCfg& c) { return c.grid_w > c.grid_h ? c.grid_w 
 // P2G with the staging chunk the launch configuration names (16, 32 or 64).
 inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     if (c.p2g_chunk == 16)
-        k_p2g<16><<<p2g_grid(c), 64, 0, s>>>(a);
+        k_p2g<16, false><<<p2g_grid(c), 64, 0, s>>>(a);
     else if (c.p2g_chunk == 64)
-        k_p2g<64><<<p2g_grid(c), 64, 0, s>>>(a);
+        k_p2g<64, false><<<p2g_grid(c), 64, 0, s>>>(a);
     else
-        k_p2g<32><<<p2g_grid(c), 64, 0, s>>>(a);
+        k_p2g<32, false><<<p2g_grid(c), 64, 0, s>>>(a);
+}
+void launch_p2g_force(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    k_p2g<32, true><<<p2g_grid(c), 64, 0, s>>>(a);
 }
 
 // G2P with or without the position projection (a branch per particle only
@@ -1804,6 +1789,21 @@ void launch_g2p(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
         launch_g2p_t<128, TRACK, PERMUTE>(a, c, s);
 }
 
+// G2P with the buffer layout's variant.
+inline void launch_g2p_any(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    if (a.cur.Fp[0]) {
+        if (a.permute)
+            launch_g2p<true, true>(a, c, s);
+        else
+            launch_g2p<true, false>(a, c, s);
+    } else {
+        if (a.permute)
+            launch_g2p<false, true>(a, c, s);
+        else
+            launch_g2p<false, false>(a, c, s);
+    }
+}
+
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev) {
     if (ev) (void)hipEventRecord(ev[0], s);
     k_bin<<<c.grid_p, 256, 0, s>>>(a);
@@ -1816,20 +1816,25 @@ void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_
     if (ev) (void)hipEventRecord(ev[2], s);
     k_grid<<<grid_grid(c), 256, 0, s>>>(a);
     if (ev) (void)hipEventRecord(ev[3], s);
-    if (a.cur.Fp[0]) {
-        if (a.permute)
-            launch_g2p<true, true>(a, c, s);
-        else
-            launch_g2p<true, false>(a, c, s);
-    } else {
-        if (a.permute)
-            launch_g2p<false, true>(a, c, s);
-        else
-            launch_g2p<false, false>(a, c, s);
-    }
+    launch_g2p_any(a, c, s);
     if (ev) (void)hipEventRecord(ev[4], s);
     k_end<<<c.grid_b, 256, 0, s>>>(a, a.permute);
     if (ev) (void)hipEventRecord(ev[5], s);
+}
+
+void launch_step_head(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    k_bin<<<c.grid_p, 256, 0, s>>>(a);
+    k_list<<<c.grid_l, 1024, 0, s>>>(a);
+    k_blocks<<<c.grid_k, 1024, 0, s>>>(a);
+    k_scatter<<<c.grid_p, 256, 0, s>>>(a);
+    k_cellsort<<<c.grid_b, 256, 0, s>>>(a);
+    launch_p2g(a, c, s);
+    k_grid<<<grid_grid(c), 256, 0, s>>>(a);
+}
+
+void launch_step_tail(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
+    launch_g2p_any(a, c, s);
+    k_end<<<c.grid_b, 256, 0, s>>>(a, a.permute);
 }
 
 void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {

@@ -24,6 +24,7 @@ struct StepArgs {
     int permute;          // 1: G2P writes every particle to nxt in sorted order (and
                           //    compacts); 0: only x v F_E J_P [F_P] S, at index i
     int project;          // some scene has colliders with position projection
+    ImpDev imp;           // semi-implicit solve (imp.gm null: explicit steps only)
 };
 
 struct LaunchCfg {
@@ -42,6 +43,12 @@ struct LaunchCfg {
 constexpr int kStepEvents = 6;
 
 void launch_step(const StepArgs& a, const LaunchCfg& c, hipStream_t s, hipEvent_t* ev);
+// The same step in two parts around the semi-implicit solve: binning, P2G and
+// the grid update (with imp.gm set: the unknowns' masses), then G2P and the end.
+void launch_step_head(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
+void launch_step_tail(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
+// P2G of the force differential only: reads imp.G instead of the force matrix.
+void launch_p2g_force(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
 void launch_setup(const StepArgs& a, const LaunchCfg& c, hipStream_t s);
 void launch_init_stress(const StepArgs& a, uint32_t first, uint32_t n, hipStream_t s);
 void launch_set_count(Counters* c, uint32_t n, hipStream_t s);

@@ -18,6 +18,48 @@ __device__ __forceinline__ double rcp_(double x) { return 1.0 / x; }
 __device__ __forceinline__ double sqrt_(double x) { return sqrt(x); }
 __device__ __forceinline__ double rsq_(double x) { return 1.0 / sqrt(x); }
 
+// Element i of a particle array as a 32-bit byte offset from the array's
+// wave-uniform base: the access takes the base from scalar registers and needs
+// no 64-bit address arithmetic per lane.  Particle indices are below 2^30
+// (the engine's capacity limit), so the offset does not wrap.
+template <typename T>
+__device__ __forceinline__ T& at(T* p, uint32_t i) {
+    static_assert(sizeof(T) == 4, "4-byte elements");
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + (i << 2));
+}
+// Component e of a vector field (x, v, F, Fp, S) of particle i: components
+// are p.stride elements apart, so a kernel needs only the first one's base.
+__device__ __forceinline__ float& comp(float* const* f, const PBuf& p, int e, uint32_t i) {
+    return at(f[0] + (size_t)e * p.stride, i);
+}
+
+// Base node of the particle's stencil, floor(x/dx) - 1 (grid.cpp:64-67).
+__device__ __forceinline__ void base_node(float inv_dx, float x0, float x1, float x2, int b[3], float f[3]) {
+    const float g0 = x0 * inv_dx, g1 = x1 * inv_dx, g2 = x2 * inv_dx;
+    const float c0 = floorf(g0), c1 = floorf(g1), c2 = floorf(g2);
+    f[0] = g0 - c0;
+    f[1] = g1 - c1;
+    f[2] = g2 - c2;
+    b[0] = (int)c0 - 1;
+    b[1] = (int)c1 - 1;
+    b[2] = (int)c2 - 1;
+}
+__device__ __forceinline__ void base_node(const SceneDev& sc, float x0, float x1, float x2, int b[3], float f[3]) {
+    base_node(sc.inv_dx, x0, x1, x2, b, f);
+}
+
+// Lame parameters of particle i: per-particle Young's modulus or the scene's.
+__device__ __forceinline__ void lame(const SceneDev& sc, const PBuf& p, uint32_t i, float& mu0, float& la0) {
+    if (sc.use_E) {
+        const float E = at(p.E, i);
+        mu0 = E * sc.mu_per_E;
+        la0 = E * sc.la_per_E;
+    } else {
+        mu0 = sc.mu0;
+        la0 = sc.lambda0;
+    }
+}
+
 // Cubic B-spline weights of the 4 stencil nodes c-1, c, c+1, c+2 for a
 // particle at fractional offset f in [0,1) inside cell c, and their
 // derivatives with respect to the particle coordinate (grid units).  These

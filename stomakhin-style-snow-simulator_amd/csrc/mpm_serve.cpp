@@ -44,6 +44,7 @@ int Engine::step_graph(int nsteps) {
         set_error("nsteps < 0");
         return MPM_ERR_ARG;
     }
+    if (implicit_active()) return step(nsteps, nullptr);  // the solve is host-driven: no graph
     int rc = setup();
     if (rc) return rc;
     for (int s = 0; s < nsteps; ++s) {
@@ -76,9 +77,8 @@ int Engine::autotune(int spc, int chosen[4]) {
         launch_ = c;
         HIP_TRY(hipEventRecord(e0, stream_));
         for (int s = 0; s < nsteps; ++s) {
-            const StepArgs a = args(kSceneActive, take_permute() ? 1 : 0);
-            launch_step(a, launch_, stream_, nullptr);
-            advance(a.permute);
+            const int r = one_step(take_permute() ? 1 : 0, nullptr, nullptr);
+            if (r) return r;
         }
         HIP_TRY(hipEventRecord(e1, stream_));
         HIP_TRY(hipEventSynchronize(e1));

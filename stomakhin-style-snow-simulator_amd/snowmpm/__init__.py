@@ -41,7 +41,8 @@ class Params(C.Structure):
     _fields_ = [("gravity", C.c_double * 3), ("dt", C.c_double), ("dx", C.c_double),
                 ("mu0", C.c_double), ("lambda0", C.c_double), ("hardening", C.c_double),
                 ("max_compression", C.c_double), ("max_stretch", C.c_double),
-                ("alpha", C.c_double), ("nu", C.c_double)]
+                ("alpha", C.c_double), ("nu", C.c_double), ("beta", C.c_double),
+                ("cr_tolerance", C.c_double), ("cr_max_iterations", C.c_int)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_}
@@ -108,6 +109,7 @@ _SIGS = {
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
     "mpm_scene_steps": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
+    "mpm_cr_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _pd, C.c_int]),
     "mpm_request_footprint": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_size_t),
                                         C.POINTER(C.c_size_t)]),
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
@@ -192,8 +194,12 @@ def lame(E, nu):
     return E / (2 * (1 + nu)), E * nu / ((1 + nu) * (1 - 2 * nu))
 
 
-def params(dt, dx, mu0, lambda0, hardening, max_compression, max_stretch, alpha, gravity=(0, -9.81, 0), nu=0.2):
+def params(dt, dx, mu0, lambda0, hardening, max_compression, max_stretch, alpha, gravity=(0, -9.81, 0), nu=0.2,
+           beta=0.0, cr_tolerance=0.0, cr_max_iterations=0):
+    """step() scalars (grid.hpp:97) + nu; beta > 0 adds the semi-implicit
+    velocity solve (cr_tolerance 0 = 1e-6, cr_max_iterations 0 = 100)."""
     p = Params()
+    p.beta, p.cr_tolerance, p.cr_max_iterations = float(beta), float(cr_tolerance), int(cr_max_iterations)
     p.gravity[:] = list(map(float, gravity))
     p.dt, p.dx, p.mu0, p.lambda0 = float(dt), float(dx), float(mu0), float(lambda0)
     p.hardening, p.max_compression, p.max_stretch = float(hardening), float(max_compression), float(max_stretch)
@@ -403,6 +409,13 @@ class Engine:
         if keep_outputs:
             res["outputs"] = outs
         return res
+
+    def cr_stats(self, sid):
+        """Iterations and |r|_M history of the scene's last semi-implicit solve."""
+        it = C.c_int()
+        res = (C.c_double * 257)()
+        _check(lib.mpm_cr_stats(self.h, int(sid), C.byref(it), res, 257))
+        return dict(iterations=it.value, residual=np.array(res[:it.value + 1]))
 
     def steps(self, sid):
         """Steps the scene has taken since it was added."""

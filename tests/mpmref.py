@@ -34,9 +34,14 @@ def orc_colliders(cs):
     return out
 
 
-def orc_params(p):
+def orc_params(p, cr_tolerance=None):
+    """The oracle's parameters for GPU parameters p; the semi-implicit solve
+    converges the oracle to cr_tolerance (default: 1e-4 of the GPU's)."""
+    tol = p.cr_tolerance if p.cr_tolerance > 0 else 1e-6
     return orc.params(p.dt, p.dx, p.mu0, p.lambda0, p.hardening, p.max_compression, p.max_stretch,
-                      p.alpha, gravity=tuple(p.gravity))
+                      p.alpha, gravity=tuple(p.gravity), beta=p.beta,
+                      cr_tolerance=cr_tolerance if cr_tolerance is not None else tol * 1e-4,
+                      cr_max_iterations=256)
 
 
 def run_oracle(sc, nsteps, V0=None, Fe=None, Fp=None, nthreads=0):
