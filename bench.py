@@ -14,10 +14,13 @@ and plastic yielding during the timed steps), the batched form (256 uneven scene
 autotuned) and the served form (Poisson arrivals at 10-200 requests/s,
 continuous batching) with p50/p99 latency, beside the plain per-scene GPU step.
 
-The seeding entry (mesh -> lattice particles, the reference's Blender add-on
-restated on the GPU) is reported beside them with its own CPU baseline.
+The semi-implicit entry (the 64k cube with the conjugate-residual velocity
+solve) and the seeding entry (mesh -> lattice particles, the reference's
+Blender add-on restated on the GPU) are reported beside them, each with its
+own CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving] [--no-seeding]
+                       [--no-implicit]
        python bench.py --batched-only --steps K   (the batched form alone, for profiling)
 """
 import argparse
@@ -50,10 +53,11 @@ STEP_KERNELS = {"k_bin": "bin", "k_scatter": "scatter", "k_cellsort": "cellsort"
 
 def write_traffic(fetch_csv, write_csv, steps):
     """profiles/traffic.json from two rocprofv3 counter runs of
-    `bench.py --steps K --no-cpu --no-serving --no-seeding --copy-calibration`
+    `bench.py --steps K --no-cpu --no-serving --no-seeding --no-implicit --copy-calibration`
     (one with --pmc FETCH_SIZE, one with --pmc WRITE_SIZE): HBM bytes per launch
-    of each step kernel, the mean over its last `steps` dispatches, each counter
-    scaled by what it reads for the 1 GiB device copy that ends the run."""
+    of each step kernel, the mean over its last `steps` dispatches (the
+    re-sort G2P: its last dispatch), each counter scaled by what it reads for
+    the 1 GiB device copy that ends the run."""
     import csv
 
     def load(path):
@@ -77,9 +81,14 @@ def write_traffic(fetch_csv, write_csv, steps):
             if base == "k_g2p":  # k_g2p<threads, track, permute, project>
                 targs = name.split("<", 1)[1].split(">", 1)[0].split(",")
                 key = "g2p_permute" if targs[2].strip() == "true" else "g2p"
+            if base == "k_p2g" and "true" in name.split("<", 1)[1].split(">", 1)[0]:
+                key = None  # k_p2g<chunk, true>: the semi-implicit solve's force differential
             if key:
                 got.setdefault(key, []).append(v * 1024.0 * factor)
-        return {k: sum(v[-steps:]) / len(v[-steps:]) for k, v in got.items()}, factor
+        # A re-sort G2P runs every 16th step: its last dispatch is the
+        # steady-state one (the first gathers the particles from input order).
+        last = {k: (v[-1:] if k == "g2p_permute" else v[-steps:]) for k, v in got.items()}
+        return {k: sum(v) / len(v) for k, v in last.items()}, factor
 
     f, ff = per_kernel(load(fetch_csv), 1 << 30)
     w, wf = per_kernel(load(write_csv), 1 << 30)
@@ -87,7 +96,7 @@ def write_traffic(fetch_csv, write_csv, steps):
     res["kernels_sha256"] = _kernels_sha()
     res["note"] = (f"HBM bytes per launch of each step kernel on the 8M-particle slab: FETCH_SIZE x {ff:.3f} + "
                    f"WRITE_SIZE x {wf:.3f} (KiB, scaled by the 1 GiB device copy), mean of the last {steps} "
-                   "dispatches")
+                   "dispatches (g2p_permute: the last re-sort dispatch, a steady-state one)")
     with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as fo:
         json.dump(res, fo, indent=1)
     print(json.dumps(res))
@@ -199,6 +208,58 @@ def slab_contact(sc, steps, warmup=10):
             "ms_per_step": wall * 1e3 / steps, "particle_steps_per_s": n * steps / wall,
             "kernel_ms": {k: round(val, 4) for k, val in kt.items() if k != "steps"},
             "yielded": int((jp < 1.0).sum()), "status": st, "fp64_fallbacks": fb}
+
+
+def semi_implicit(seed, no_cpu, steps=20, warmup=2, beta=1.0, tol=1e-6, E=1e7, dt=5e-4):
+    """The 64,000-particle cube (falling at 2 m/s onto the ground plane) made
+    stiff (E = 1e7) and stepped at dt = 5e-4, about 4x the explicit stability
+    limit dx / sqrt(E / rho), with the semi-implicit velocity solve, beta = 1,
+    conjugate residuals to |r|_M <= 1e-6 |v*|_M: step time, CR iterations per
+    step, and the FP64 oracle's step with the same solve and tolerance as the
+    CPU baseline."""
+    import numpy as np
+    import snowmpm
+
+    sc = snowmpm.generate_scene(0, seed=seed)
+    prm = type(sc["params"]).from_buffer_copy(sc["params"])
+    prm.beta, prm.cr_tolerance, prm.cr_max_iterations = beta, tol, 200
+    prm.mu0, prm.lambda0 = snowmpm.lame(E, prm.nu)
+    prm.dt = dt
+    n = sc["x"].shape[0]
+    iters = []
+    with snowmpm.Engine(capacity=n + 1024) as eng:
+        sid = eng.add_scene(prm, sc["colliders"], sc["x"], sc["v"], sc["m"])
+        eng.setup()
+        eng.step(warmup)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            eng.step(1)
+            iters.append(eng.cr_stats(sid)["iterations"])
+        wall = time.perf_counter() - t0
+        st, _ = eng.status(sid)
+    out = {"workload": f"cube {n} particles falling onto the ground plane, E {E:g}, dx 0.02, dt {dt:g}, "
+                       f"beta {beta:g}, CR to {tol:g} |v*|_M, steps {warmup + 1}-{warmup + steps}",
+           "ms_per_step": wall * 1e3 / steps, "particle_steps_per_s": n * steps / wall,
+           "cr_iterations_mean": float(np.mean(iters)), "cr_iterations_max": int(max(iters)), "status": st}
+    if not no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import mpmref  # the oracle driver, used here only as the timed CPU baseline
+        import oracle as orc
+        info = _host_info()
+        threads = min(info["affinity"], int(info["OMP_NUM_THREADS"] or info["affinity"]))
+        st = orc.State(sc["x"], sc["v"], sc["m"])
+        st.setup(prm.dx)
+        cs = mpmref.orc_colliders(sc["colliders"])
+        op = mpmref.orc_params(prm, cr_tolerance=tol)
+        st.run(cs, op, 1, nthreads=threads)
+        t0 = time.perf_counter()
+        st.run(cs, op, 2, nthreads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * 2 / dt, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"same cube, 1 warm-up + 2 timed steps, {dt:.1f} s; FP64 oracle, same "
+                                         "solve and tolerance"}
+    return out
 
 
 def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(10, 25, 50, 100, 200),
@@ -348,6 +409,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true",
                     help="skip the batched / served measurements and the slab-contact run")
     ap.add_argument("--no-seeding", action="store_true", help="skip the mesh-seeding measurement")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the semi-implicit measurement")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched form (256 scenes), --steps graph-replayed steps each (for profiling)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
@@ -452,6 +514,8 @@ def main():
     if not args.no_serving:
         out["slab_contact"] = slab_contact(sc, args.steps)
         out["batched"] = batched_and_served(args.seed)
+    if not args.no_implicit:
+        out["semi_implicit"] = semi_implicit(args.seed, args.no_cpu)
     if not args.no_seeding:
         out["seeding"] = seeding(args.no_cpu)
     if args.copy_calibration:
