@@ -22,6 +22,7 @@ own CPU baseline.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving] [--no-seeding]
                        [--no-implicit]
        python bench.py --batched-only --steps K   (the batched form alone, for profiling)
+       python bench.py --issue-probe               (MFMA/VALU issue rates behind the P2G bound)
 """
 import argparse
 import json
@@ -157,6 +158,24 @@ def cpu_baseline(sc, seed, scale=1.0, steps=3):
                      f"{500 * dt / steps / 3600:.2f} h (extrapolated)"}
     out.update(info)
     return out
+
+
+def issue_probe():
+    """SIMD cycles per iteration of 4 independent FP32 MFMAs, of 4 K FP32
+    FMAs and of both interleaved, at 1, 2 and 4 waves per SIMD
+    (mpm_probe_issue): if the matrix core and the VALU issued side by side
+    the mixed loop would cost the larger of the two, not their sum."""
+    import snowmpm
+    rows = []
+    for w in (1, 2, 4):
+        for k in (1, 2, 4, 8, 16):
+            r = snowmpm.probe_issue(w, k)
+            rows.append({"waves_per_simd": w, "valu_per_mfma": k, "mfma_cycles": round(r["mfma"], 1),
+                         "valu_cycles": round(r["valu"], 1), "mixed_cycles": round(r["mixed"], 1),
+                         "mixed_over_sum": round(r["mixed"] / (r["mfma"] + r["valu"]), 3),
+                         "mixed_over_max": round(r["mixed"] / max(r["mfma"], r["valu"]), 3),
+                         "kernel_us": [round(u, 1) for u in r["us"]]})
+    return rows
 
 
 def copy_calibration(nbytes=1 << 30):
@@ -414,6 +433,8 @@ def main():
                     help="only the batched form (256 scenes), --steps graph-replayed steps each (for profiling)")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 counter CSVs (no GPU work)")
+    ap.add_argument("--issue-probe", action="store_true",
+                    help="only the MFMA/VALU issue-rate probe (one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -428,6 +449,9 @@ def main():
 
     import snowmpm
 
+    if args.issue_probe:
+        print(json.dumps({"issue_probe": issue_probe()}))
+        return
     if args.batched_only:
         print(json.dumps({"batched": batched_and_served(args.seed, steps=args.steps, batched_only=True)}))
         return

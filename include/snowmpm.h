@@ -164,6 +164,14 @@ int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]);
  * [bin+sort, p2g, grid, g2p(+fp64 fix-up), end]; *nsteps = steps averaged. */
 int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
 
+/* Issue-rate probe (no counterpart in the reference; a measurement for the
+ * P2G bound): on `device`, waves_per_simd waves per SIMD each run
+ * `iterations` iterations of 4 independent v_mfma_f32_16x16x4f32 (mode 0),
+ * of 4 * valu_per_mfma FP32 FMAs (mode 1, valu_per_mfma in 1, 2, 4, 8, 16),
+ * or of both interleaved (mode 2).  out[mode] = shader-clock cycles per
+ * iteration of the slowest wave, out[3 + mode] = kernel microseconds. */
+int mpm_probe_issue(int device, int waves_per_simd, int valu_per_mfma, int iterations, double out[6]);
+
 /* The same steps replayed from a captured hipGraph (one graph per buffer
  * parity, re-captured only when the launch setup changes); the device-side
  * particle/block counts make one graph valid for any set of resident scenes.

@@ -75,13 +75,15 @@ __global__ void k_set_count(Counters* c, uint32_t n) {
 
 // Per particle: hash-insert the home block, count the particle into its cell.
 // Consecutive particles mostly share a block and a cell (the buffer is sorted
-// by the previous step), so both are aggregated per wave: one lane inserts
-// each distinct block key, one lane bumps each distinct cell counter by the
-// number of lanes in it, and lanes take consecutive ranks.  A wave takes two
-// sets of 64 consecutive particles per pass, whose hash lookups and atomics
-// are in flight together (set 0 elects the lowest lane of each distinct key,
-// set 1 the highest, so a lane seldom serves both), and the positions of the
-// next pass are loaded meanwhile.
+// by the previous step), so both are aggregated per wave over runs of equal
+// keys in consecutive lanes: one lane of each run inserts the block key, one
+// lane of each run bumps the cell counter by the run's length, and the run's
+// lanes take consecutive ranks.  A key that appears in several runs (particles
+// out of sorted order) is inserted or counted once per run, which is still
+// exact.  A wave takes two sets of 64 consecutive particles per pass, whose
+// hash lookups and atomics are in flight together (set 0's runs are served by
+// their first lane, set 1's by their last, so a lane seldom serves both), and
+// the positions of the next pass are loaded meanwhile.
 struct BinItem {
     uint64_t key;
     uint32_t l, sid;
@@ -128,38 +130,22 @@ __device__ __forceinline__ BinItem bin_item(const StepArgs& a, const SceneCache&
     it.want = true;
     return it;
 }
-// Leader lane of this lane's 64-bit value among the lanes with want set:
-// the lowest (HIGH = false) or highest lane holding the same value.
-template <bool HIGH>
-__device__ __forceinline__ int bin_leader64(uint64_t v, bool want) {
-    int leader = (int)(threadIdx.x & 63);
-    uint64_t pending = __ballot(want);
-    while (pending) {
-        const int l = HIGH ? 63 - __clzll((long long)pending) : __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lo = __shfl((uint32_t)v, l), hi = __shfl((uint32_t)(v >> 32), l);
-        const bool match = want && v == (((uint64_t)hi << 32) | lo);
-        if (match) leader = l;
-        pending &= ~__ballot(match);
-    }
-    return leader;
+// Runs of equal keys among the lanes with want set.  heads: the lanes that
+// differ from the lane below (or whose lane below has no key), tails: the
+// lanes whose lane above starts a run or has no key.  Per lane, the first
+// (hl) and last (tl) lane of its run; le / ge are the masks of the lanes at
+// or below / at or above this lane.
+struct Run {
+    int hl, tl;
+};
+__device__ __forceinline__ int lowest_lane(uint64_t m) { return __ffsll((unsigned long long)m) - 1; }
+__device__ __forceinline__ int highest_lane(uint64_t m) { return 63 - __clzll((long long)m); }
+__device__ __forceinline__ Run bin_runs(bool head, bool want, uint64_t le, uint64_t ge) {
+    const uint64_t H = __ballot(head), W = __ballot(want);
+    const uint64_t T = W & ((H >> 1) | ~(W >> 1));
+    return Run{highest_lane(H & le), lowest_lane(T & ge)};
 }
-template <bool HIGH>
-__device__ __forceinline__ int bin_leader32(uint32_t v, bool want, uint64_t& group) {
-    int leader = (int)(threadIdx.x & 63);
-    group = 0;
-    uint64_t pending = __ballot(want);
-    while (pending) {
-        const int l = HIGH ? 63 - __clzll((long long)pending) : __ffsll((unsigned long long)pending) - 1;
-        const bool match = want && v == __shfl(v, l);
-        const uint64_t mm = __ballot(match);
-        if (match) {
-            leader = l;
-            group = mm;
-        }
-        pending &= ~mm;
-    }
-    return leader;
-}
+__device__ __forceinline__ uint32_t lane_below(uint32_t v) { return __shfl_up(v, 1, 64); }
 
 __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     const uint32_t n = a.cnt->n_particles;
@@ -170,7 +156,7 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
         a.cnt->n_fallback = 0;
     }
     const uint32_t lane = threadIdx.x & 63;
-    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const uint64_t le = lane == 63 ? ~0ull : (2ull << lane) - 1ull, ge = ~((1ull << lane) - 1ull);
     const uint32_t stride = gridDim.x * blockDim.x * 2;
     uint32_t wbase = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) * 2;
     uint32_t nsid[2] = {0u, 0u};
@@ -198,9 +184,15 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
         for (int s = 0; s < 2; ++s)
             it[s] = bin_item(a, scc, wbase + 64 * s + lane, n, nsid[s], nx[s][0], nx[s][1], nx[s][2]);
         load(wbase + stride);
-        // block keys: one insert per distinct key and set
-        const int kl0 = bin_leader64<false>(it[0].key, it[0].want);
-        const int kl1 = bin_leader64<true>(it[1].key, it[1].want);
+        // block keys: one insert per run (set 0 by its first lane, set 1 by its last)
+        Run kr[2];
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const uint32_t plo = lane_below((uint32_t)it[s].key), phi = lane_below((uint32_t)(it[s].key >> 32));
+            const bool head = it[s].want && (lane == 0 || (((uint64_t)phi << 32) | plo) != it[s].key);
+            kr[s] = bin_runs(head, it[s].want, le, ge);
+        }
+        const int kl0 = kr[0].hl, kl1 = kr[1].tl;
         const bool do0 = it[0].want && kl0 == (int)lane, do1 = it[1].want && kl1 == (int)lane;
         uint32_t s0 = kInvalid, s1 = kInvalid;
         bool ins;
@@ -213,34 +205,36 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
         }
         if (do0 && s0 == kInvalid) flag(a.st, it[0].sid, kStatusHashFull);
         if (do1 && s1 == kInvalid) flag(a.st, it[1].sid, kStatusHashFull);
-        const uint32_t slot0 = it[0].want ? __shfl(s0, kl0) : kInvalid;
-        const uint32_t slot1 = it[1].want ? __shfl(s1, kl1) : kInvalid;
-        // cells: one counter bump per distinct cell and set
-        const bool h0 = slot0 != kInvalid, h1 = slot1 != kInvalid;
+        const uint32_t slot0 = __shfl(s0, it[0].want ? kl0 : (int)lane);
+        const uint32_t slot1 = __shfl(s1, it[1].want ? kl1 : (int)lane);
+        // cells: one counter bump per run of equal cell keys, by the run's length
+        const bool h0 = it[0].want && slot0 != kInvalid, h1 = it[1].want && slot1 != kInvalid;
         const uint32_t ck0 = h0 ? (slot0 << 6) | it[0].l : kInvalid;
         const uint32_t ck1 = h1 ? (slot1 << 6) | it[1].l : kInvalid;
-        uint64_t g0, g1;
-        const int cl0 = bin_leader32<false>(ck0, h0, g0);
-        const int cl1 = bin_leader32<true>(ck1, h1, g1);
+        const uint32_t pc0 = lane_below(ck0), pc1 = lane_below(ck1);
+        const Run cr0 = bin_runs(h0 && (lane == 0 || pc0 != ck0), h0, le, ge);
+        const Run cr1 = bin_runs(h1 && (lane == 0 || pc1 != ck1), h1, le, ge);
+        const int cl0 = cr0.hl, cl1 = cr1.tl;
         const bool c0 = h0 && cl0 == (int)lane, c1 = h1 && cl1 == (int)lane;
         uint32_t b0 = 0u, b1 = 0u;
         if (c0 && c1) {
-            b0 = atomicAdd(&a.g.cellcnt[ck0], (uint32_t)__popcll(g0));
-            b1 = atomicAdd(&a.g.cellcnt[ck1], (uint32_t)__popcll(g1));
+            b0 = atomicAdd(&a.g.cellcnt[ck0], (uint32_t)(cr0.tl - cr0.hl + 1));
+            b1 = atomicAdd(&a.g.cellcnt[ck1], (uint32_t)(cr1.tl - cr1.hl + 1));
         } else if (c0 || c1) {
-            const uint32_t b = atomicAdd(&a.g.cellcnt[c0 ? ck0 : ck1], (uint32_t)__popcll(c0 ? g0 : g1));
+            const uint32_t b = atomicAdd(&a.g.cellcnt[c0 ? ck0 : ck1],
+                                         (uint32_t)(c0 ? cr0.tl - cr0.hl + 1 : cr1.tl - cr1.hl + 1));
             (c0 ? b0 : b1) = b;
         }
-        b0 = __shfl(b0, cl0);
-        b1 = __shfl(b1, cl1);
+        b0 = __shfl(b0, h0 ? cl0 : (int)lane);
+        b1 = __shfl(b1, h1 ? cl1 : (int)lane);
         const uint32_t i0 = wbase + lane, i1 = wbase + 64 + lane;
         if (i0 < n) {
             a.bin.pkey[i0] = ck0;
-            a.bin.prank[i0] = h0 ? b0 + (uint32_t)__popcll(g0 & lt_mask) : 0u;
+            a.bin.prank[i0] = h0 ? b0 + (lane - (uint32_t)cr0.hl) : 0u;
         }
         if (i1 < n) {
             a.bin.pkey[i1] = ck1;
-            a.bin.prank[i1] = h1 ? b1 + (uint32_t)__popcll(g1 & lt_mask) : 0u;
+            a.bin.prank[i1] = h1 ? b1 + (lane - (uint32_t)cr1.hl) : 0u;
         }
     }
 }

@@ -108,6 +108,7 @@ _SIGS = {
     "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
+    "mpm_probe_issue": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pd]),
     "mpm_scene_steps": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     "mpm_cr_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _pd, C.c_int]),
     "mpm_request_footprint": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_size_t),
@@ -245,6 +246,15 @@ def export_frame(directory, frame_no, x):
     """export_particles (export.cpp:10-22) into directory/frame_%06d.csv."""
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
     _check(lib.mpm_export_frame(os.fsencode(directory), int(frame_no), x.shape[0], _dptr(x)))
+
+
+def probe_issue(waves_per_simd, valu_per_mfma, iterations=4096, device=0):
+    """SIMD cycles per iteration of 4 independent FP32 MFMAs (mfma), of
+    4 * valu_per_mfma FP32 FMAs (valu) and of both interleaved (mixed), with
+    the kernel times in microseconds (see mpm_probe_issue)."""
+    out = (C.c_double * 6)()
+    _check(lib.mpm_probe_issue(int(device), int(waves_per_simd), int(valu_per_mfma), int(iterations), out))
+    return dict(mfma=out[0], valu=out[1], mixed=out[2], us=(out[3], out[4], out[5]))
 
 
 def seed_mesh(verts, tris, spacing, device=0):

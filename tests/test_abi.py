@@ -74,3 +74,12 @@ def test_batched_distribution():
     ns = [snowmpm.generate_scene(3, seed=3, index=i)["x"].shape[0] for i in range(64)]
     assert min(ns) >= 300 and max(ns) <= 260_000
     assert np.median(ns) > 3000
+
+
+def test_probe_rejects_bad_arguments_without_a_gpu():
+    lib = ctypes.CDLL(snowmpm._LIB)
+    out = (ctypes.c_double * 6)()
+    assert lib.mpm_probe_issue(0, 0, 4, 16, out) == snowmpm.ERR_ARG      # no waves
+    assert lib.mpm_probe_issue(0, 1, 3, 16, out) == snowmpm.ERR_ARG      # K not in 1, 2, 4, 8, 16
+    assert lib.mpm_probe_issue(0, 1, 4, 0, out) == snowmpm.ERR_ARG       # no iterations
+    assert lib.mpm_probe_issue(0, 1, 4, 16, None) == snowmpm.ERR_ARG
