@@ -294,15 +294,29 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
     const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t b0 = blockIdx.x * 16; b0 < nh; b0 += gridDim.x * 16) {
+    const uint32_t stride = gridDim.x * 16;
+    // The next block's slot, cell counts and key are loaded while this block
+    // is handled (its hash inserts and counter updates wait on memory).
+    uint32_t nslot = 0u, nc = 0u;
+    uint64_t nkey = 0ull;
+    auto records = [&](uint32_t bb) {
+        nslot = a.g.home_list[bb];
+        nc = a.g.cellcnt[(size_t)nslot * 64 + lane];
+        if (lane > 0 && lane < 8) nkey = a.g.keys[nslot];
+    };
+    if (blockIdx.x * 16 + wid < nh) records(blockIdx.x * 16 + wid);
+    for (uint32_t b0 = blockIdx.x * 16; b0 < nh; b0 += stride) {
         const uint32_t b = b0 + wid;
         const bool valid = b < nh;
-        const uint32_t slot = valid ? a.g.home_list[b] : 0u;
+        const uint32_t slot = valid ? nslot : 0u;
+        const uint32_t c = nc;
+        const uint64_t key = nkey;
+        const uint32_t bn = b + stride;
+        const uint32_t pslot = bn < nh ? a.g.home_list[bn] : 0u;  // the next block's slot, in flight meanwhile
         uint32_t total = 0;
         uint32_t ns = slot;
         bool ins = false;
         if (valid) {
-            const uint32_t c = a.g.cellcnt[(size_t)slot * 64 + lane];
             uint32_t s = c;
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) {
@@ -315,7 +329,7 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                 if (lane != 0) {
                     uint32_t sid;
                     int bx, by, bz;
-                    unpack_key(a.g.keys[slot], sid, bx, by, bz);
+                    unpack_key(key, sid, bx, by, bz);
                     ns = hash_insert(a.g, pack_key(sid, bx + (int)(lane & 1), by + (int)((lane >> 1) & 1), bz + (int)(lane >> 2)), ins);
                     if (ns == kInvalid) {
                         flag(a.st, sid, kStatusHashFull);
@@ -330,6 +344,11 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                 // holds partial sums for it; only H writes entry t of H + t.
                 if (b < a.g.max_home) a.g.rev[(size_t)ns * 8 + lane] = b;
             }
+        }
+        if (bn < nh) {  // the next block's cell counts and key
+            nslot = pslot;
+            nc = a.g.cellcnt[(size_t)pslot * 64 + lane];
+            if (lane > 0 && lane < 8) nkey = a.g.keys[pslot];
         }
         // Neighbour blocks created here are appended to the all-blocks list
         // with one counter update per workgroup (a counter bumped by every
@@ -421,9 +440,24 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
         if (nb <= (uint32_t)kSortCap) {
             uint32_t* P = kp[wid];
             uint32_t* I = ki[wid];
-            for (uint32_t t = lane; t < nb; t += 64) {
-                P[t] = a.bin.spid[off + t];
-                I[t] = a.bin.sidx[off + t];
+            // all loads of a batch of 8 x 64 particles in flight before any
+            // is written to LDS (one memory round trip per batch)
+            for (uint32_t t0 = 0; t0 < nb; t0 += 8 * 64) {
+                uint32_t pv[8], iv[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t t = t0 + lane + 64 * k;
+                    pv[k] = t < nb ? a.bin.spid[off + t] : 0u;
+                    iv[k] = t < nb ? a.bin.sidx[off + t] : 0u;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t t = t0 + lane + 64 * k;
+                    if (t < nb) {
+                        P[t] = pv[k];
+                        I[t] = iv[k];
+                    }
+                }
             }
             __asm__ __volatile__("s_waitcnt lgkmcnt(0)" ::: "memory");
             bool moved = false;
