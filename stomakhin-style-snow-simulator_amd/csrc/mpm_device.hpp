@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stddef.h>
 
 namespace snowmpm {
 
@@ -102,6 +103,9 @@ struct Counters {
     uint32_t global_status;
     uint32_t pad_[2];
 };
+// k_blocks bumps n_extra and cursor with one 64-bit atomic.
+static_assert(offsetof(Counters, n_extra) % 8 == 0 && offsetof(Counters, cursor) == offsetof(Counters, n_extra) + 4,
+              "n_extra and cursor form one aligned 64-bit word");
 
 struct GridDev {
     uint64_t* keys;        // [H] hash keys, kEmptyKey when free

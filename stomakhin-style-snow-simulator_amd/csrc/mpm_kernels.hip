@@ -368,8 +368,14 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
                 run += v;
                 erun += e;
             }
-            base_s = run ? atomicAdd(&a.cnt->cursor, run) : 0u;
-            ext_s = erun ? atomicAdd(&a.cnt->n_extra, erun) : 0u;
+            // one 64-bit atomic for both counters (n_extra low, cursor high;
+            // neither sum reaches 2^32, so no carry crosses between them)
+            const unsigned long long old =
+                (run | erun) ? atomicAdd(reinterpret_cast<unsigned long long*>(&a.cnt->n_extra),
+                                         ((unsigned long long)run << 32) | erun)
+                             : 0ull;
+            ext_s = (uint32_t)old;
+            base_s = (uint32_t)(old >> 32);
         }
         __syncthreads();
         if (valid && lane == 0) {
