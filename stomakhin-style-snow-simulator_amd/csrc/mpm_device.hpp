@@ -150,11 +150,11 @@ struct ImpDev {
     float* G;               // [9][N] per-particle -V0 dP F^T for the force differential
     uint32_t* ord;          // [nblocks] slots of all blocks in key order
     uint64_t* skey;         // [nblocks] their keys
-    double* dp;             // [nblocks*2] per-block dot-product partials (key order)
+    double* dp;             // [nblocks*4] per-block dot-product partials (key order)
     uint32_t* seg;          // [2*kMaxScenes] per scene: first, end index into ord
     CRScene* cr;            // [kMaxScenes]
     double* hist;           // [kMaxScenes*kCRHist] |r|_M per iteration of the last solve
-    uint32_t* nactive;      // scenes still iterating
+    uint32_t* nactive;      // [kCRHist + 1] scenes still iterating after each reduction of a solve
 };
 constexpr int kCRHist = 257;
 

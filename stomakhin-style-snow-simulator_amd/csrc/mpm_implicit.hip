@@ -87,9 +87,10 @@ __global__ __launch_bounds__(256) void k_imp_init(StepArgs a, uint32_t n) {
 
 // The force differential summed from the home blocks' partial sums (channels
 // 4..6, layout at k_grid), then Y = Q - beta dt df / m on the unknowns.
-//   mode 0: Q = X:  R = v* - Y;            dots <R,R>, <v*,v*>
-//   mode 1: Q = R:  AR = AP = Y, P = R;    dots <R,AR>, <AR,AR>
-//   mode 2: Q = R:  AR = Y;                dot  <R,AR>
+//   mode 0: Q = X:  R = v* - Y;            dots <R,R>, <v*,v*>        (slots 0, 1)
+//   mode 1: Q = R:  AR = AP = Y, P = R;    dots <R,AR>, <AR,AR>       (slots 0, 1)
+//   mode 2: Q = R:  AR = Y;                dots <R,AR>, <AR,AR>, <AR,AP> (slots 1, 2, 3)
+// (per block 4 partial slots in dp; slot 0 of mode 2 is k_imp_vec's <R,R>)
 __global__ __launch_bounds__(256) void k_imp_apply(StepArgs a, uint32_t n, int mode) {
     const int lane = (int)(threadIdx.x & 63);
     const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void k_imp_apply(StepArgs a, uint32_t n, int m
         const float m = a.imp.gm[(size_t)slot * 64 + lane];
         const float* Q = mode == 0 ? X : R;
         const float s = m > 0.0f ? sc.beta * sc.dt * rcp_(m) : 0.0f;
-        double d0 = 0.0, d1 = 0.0;
+        double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float q = *vec_at(const_cast<float*>(Q), slot, c, lane);
@@ -134,27 +135,36 @@ __global__ __launch_bounds__(256) void k_imp_apply(StepArgs a, uint32_t n, int m
             } else {
                 *vec_at(AR, slot, c, lane) = y;
                 d0 += (double)m * (double)q * (double)y;
+                d1 += (double)m * (double)y * (double)y;
                 if (mode == 1) {
                     *vec_at(AP, slot, c, lane) = y;
                     *vec_at(P, slot, c, lane) = q;
-                    d1 += (double)m * (double)y * (double)y;
+                } else {
+                    d2 += (double)m * (double)y * (double)*vec_at(AP, slot, c, lane);
                 }
             }
         }
         d0 = wave_sum(d0);
         d1 = wave_sum(d1);
+        if (mode == 2) d2 = wave_sum(d2);
         if (lane == 0) {
-            a.imp.dp[2 * (size_t)b] = d0;
-            a.imp.dp[2 * (size_t)b + 1] = d1;
+            double* o = a.imp.dp + 4 * (size_t)b;
+            if (mode == 2) {
+                o[1] = d0;
+                o[2] = d1;
+                o[3] = d2;
+            } else {
+                o[0] = d0;
+                o[1] = d1;
+            }
         }
     }
 }
 
-// CR vector updates with the scene's scalars.
-//   mode 0: X += alpha P, R -= alpha AP;   dot <R,R>
-//   mode 1: P = R + beta P, AP = AR + beta AP;   dot <AP,AP>
-// and mode 2: the solution into the grid: v = X and u = u* + (X - v*) on the
-// unknowns.
+// CR vector updates with the scene's scalars, mode 0: the direction update
+// of the previous iteration, P = R + beta P and AP = AR + beta AP, then
+// X += alpha P and R -= alpha AP; dot <R,R> (slot 0).  Mode 2: the solution
+// into the grid: v = X and u = u* + (X - v*) on the unknowns.
 __global__ __launch_bounds__(256) void k_imp_vec(StepArgs a, uint32_t n, int mode) {
     const int lane = (int)(threadIdx.x & 63);
     float* X = a.imp.vec[0];
@@ -166,7 +176,6 @@ __global__ __launch_bounds__(256) void k_imp_vec(StepArgs a, uint32_t n, int mod
         const uint32_t slot = a.imp.ord[b];
         const uint32_t sid = (uint32_t)(a.imp.skey[b] >> 51);
         const float m = a.imp.gm[(size_t)slot * 64 + lane];
-        double d0 = 0.0;
         if (mode == 2) {
             if (m > 0.0f)
 #pragma unroll
@@ -178,27 +187,26 @@ __global__ __launch_bounds__(256) void k_imp_vec(StepArgs a, uint32_t n, int mod
                 }
             continue;
         }
-        const float coef = (float)(mode == 0 ? a.imp.cr[sid].alpha : a.imp.cr[sid].beta);
+        const float alpha = (float)a.imp.cr[sid].alpha, beta = (float)a.imp.cr[sid].beta;
+        double d0 = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            if (mode == 0) {
-                float* x = vec_at(X, slot, c, lane);
-                float* r = vec_at(R, slot, c, lane);
-                *x = *x + coef * *vec_at(P, slot, c, lane);
-                const float rn = *r - coef * *vec_at(AP, slot, c, lane);
-                *r = rn;
-                d0 += (double)m * (double)rn * (double)rn;
-            } else {
-                float* p = vec_at(P, slot, c, lane);
-                float* ap = vec_at(AP, slot, c, lane);
-                *p = *vec_at(R, slot, c, lane) + coef * *p;
-                const float apn = *vec_at(AR, slot, c, lane) + coef * *ap;
-                *ap = apn;
-                d0 += (double)m * (double)apn * (double)apn;
-            }
+            float* p = vec_at(P, slot, c, lane);
+            float* ap = vec_at(AP, slot, c, lane);
+            float* x = vec_at(X, slot, c, lane);
+            float* r = vec_at(R, slot, c, lane);
+            const float rc = *r;
+            const float pn = rc + beta * *p;
+            const float apn = *vec_at(AR, slot, c, lane) + beta * *ap;
+            *p = pn;
+            *ap = apn;
+            *x = *x + alpha * pn;
+            const float rn = rc - alpha * apn;
+            *r = rn;
+            d0 += (double)m * (double)rn * (double)rn;
         }
         d0 = wave_sum(d0);
-        if (lane == 0) a.imp.dp[2 * (size_t)b] = d0;
+        if (lane == 0) a.imp.dp[4 * (size_t)b] = d0;
     }
 }
 
@@ -206,23 +214,35 @@ __global__ __launch_bounds__(256) void k_imp_vec(StepArgs a, uint32_t n, int mod
 // blocks in key order (lane-strided, then a fixed butterfly), and the CR
 // scalars updated:
 //   kInit: rr, bb from mode-0 apply; active if rr > tol^2 bb
-//   kRho0: rho = <R,AR>, den = <AR,AR>, alpha = rho / den
-//   kRR:   rr = <R,R> after an update; stop at the tolerance or max iterations
-//   kRho:  beta = <R,AR>_new / rho, rho = <R,AR>_new
-//   kDen:  den = <AP,AP>, alpha = rho / den
-enum { kInit = 0, kRho0 = 1, kRR = 2, kRho = 3, kDen = 4 };
-__global__ __launch_bounds__(256) void k_imp_reduce(StepArgs a, uint32_t nscenes, int mode) {
+//   kRho0: rho = <R,AR>, den = <AR,AR>, alpha = rho / den, beta = 0
+//   kIter: rr = <R,R> after k_imp_vec's update (stop at the tolerance or the
+//          iteration cap); then rho' = <R,AR>, beta = rho' / rho and, for the
+//          next direction AP' = AR + beta AP,
+//          den = <AP',AP'> = <AR,AR> + 2 beta <AR,AP> + beta^2 <AP,AP>,
+//          alpha = rho' / den (both 0 once the scene stops)
+// Scenes still iterating are counted into nactive[slot].
+enum { kInit = 0, kRho0 = 1, kIter = 2 };
+__global__ __launch_bounds__(256) void k_imp_reduce(StepArgs a, uint32_t nscenes, int mode, int slot) {
     const int lane = (int)(threadIdx.x & 63);
     const uint32_t sid = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (sid >= nscenes) return;
     const uint32_t b0 = a.imp.seg[2 * sid], b1 = a.imp.seg[2 * sid + 1];
-    double s0 = 0.0, s1 = 0.0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
     for (uint32_t b = b0 + lane; b < b1; b += 64) {
-        s0 += a.imp.dp[2 * (size_t)b];
-        s1 += a.imp.dp[2 * (size_t)b + 1];
+        const double* d = a.imp.dp + 4 * (size_t)b;
+        s0 += d[0];
+        s1 += d[1];
+        if (mode == kIter) {
+            s2 += d[2];
+            s3 += d[3];
+        }
     }
     s0 = wave_sum(s0);
     s1 = wave_sum(s1);
+    if (mode == kIter) {
+        s2 = wave_sum(s2);
+        s3 = wave_sum(s3);
+    }
     if (lane != 0) return;
     CRScene& c = a.imp.cr[sid];
     const SceneDev& sc = a.scenes[sid];
@@ -234,24 +254,31 @@ __global__ __launch_bounds__(256) void k_imp_reduce(StepArgs a, uint32_t nscenes
         c.alpha = c.beta = 0.0;
         hist[0] = sqrt(s0);
         c.active = (b1 > b0 && sc.beta > 0.0f && s1 > 0.0 && s0 > (double)sc.cr_tol2 * s1 && sc.cr_maxit > 0) ? 1 : 0;
-    } else if (mode == kRho0 || mode == kDen) {
-        if (mode == kRho0) c.rho = s0;
-        c.den = mode == kRho0 ? s1 : s0;
+    } else if (mode == kRho0) {
+        c.rho = s0;
+        c.den = s1;
+        c.beta = 0.0;
         c.alpha = (c.active && c.den > 0.0 && c.rho != 0.0) ? c.rho / c.den : 0.0;
         if (c.alpha == 0.0) c.active = 0;
-    } else if (mode == kRR) {
+    } else {  // kIter
         if (c.active) {
             c.rr = s0;
             c.iters += 1;
             hist[c.iters] = sqrt(s0);
             if (s0 <= (double)sc.cr_tol2 * c.bb || c.iters >= sc.cr_maxit) c.active = 0;
         }
-        c.alpha = 0.0;
-    } else {  // kRho
-        c.beta = (c.active && c.rho != 0.0) ? s0 / c.rho : 0.0;
-        if (c.active) c.rho = s0;
+        if (c.active && c.rho != 0.0) {
+            const double beta = s1 / c.rho;
+            const double den = s2 + 2.0 * beta * s3 + beta * beta * c.den;
+            c.beta = beta;
+            c.rho = s1;
+            c.den = den;
+            c.alpha = (den > 0.0 && s1 != 0.0) ? s1 / den : 0.0;
+            if (c.alpha == 0.0) c.active = 0;
+        }
+        if (!c.active) c.alpha = c.beta = 0.0;
     }
-    if (c.active) atomicAdd(a.imp.nactive, 1u);
+    if (c.active) atomicAdd(a.imp.nactive + slot, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -454,11 +481,11 @@ int Engine::implicit_alloc() {
     if ((rc = dalloc((void**)&imp_.G, N * 9 * 4))) return rc;
     if ((rc = dalloc((void**)&imp_.ord, H * 4))) return rc;
     if ((rc = dalloc((void**)&imp_.skey, H * 8))) return rc;
-    if ((rc = dalloc((void**)&imp_.dp, H * 2 * 8))) return rc;
+    if ((rc = dalloc((void**)&imp_.dp, H * 4 * 8))) return rc;
     if ((rc = dalloc((void**)&imp_.seg, 2 * kMaxScenes * 4))) return rc;
     if ((rc = dalloc((void**)&imp_.cr, kMaxScenes * sizeof(CRScene)))) return rc;
     if ((rc = dalloc((void**)&imp_.hist, (size_t)kMaxScenes * kCRHist * 8))) return rc;
-    if ((rc = dalloc((void**)&imp_.nactive, 4))) return rc;
+    if ((rc = dalloc((void**)&imp_.nactive, (kCRHist + 1) * 4))) return rc;
     if ((rc = dalloc((void**)&imp_kin_, H * 8))) return rc;
     if ((rc = dalloc((void**)&imp_vin_, H * 4))) return rc;
     imp_sort_bytes_ = 0;
@@ -495,19 +522,18 @@ int Engine::implicit_solve(const StepArgs& a) {
     int maxit = 0;
     for (size_t k = 0; k < scenes_.size(); ++k)
         if (scenes_[k].used && h_scenes_[k].beta > 0.0f) maxit = std::max(maxit, h_scenes_[k].cr_maxit);
-    auto reduce = [&](int mode) -> int {
-        HIP_TRY(hipMemsetAsync(imp_.nactive, 0, 4, s));
-        k_imp_reduce<<<gs, 256, 0, s>>>(a, nsc, mode);
-        return MPM_OK;
-    };
+    // nactive[k]: scenes still iterating after the k-th reduction of this
+    // solve (cleared once here, so the loop has no memory-set nodes)
+    HIP_TRY(hipMemsetAsync(imp_.nactive, 0, (kCRHist + 1) * 4, s));
+    auto reduce = [&](int mode, int slot) { k_imp_reduce<<<gs, 256, 0, s>>>(a, nsc, mode, slot); };
     auto apply = [&](const float* q, int mode) {
         k_imp_dF<<<gw, kImpThreads, 0, s>>>(a, q);
         launch_p2g_force(a, launch_, s);
         k_imp_apply<<<gb, 256, 0, s>>>(a, n, mode);
     };
-    auto any_active = [&](bool* any) -> int {
+    auto any_active = [&](int slot, bool* any) -> int {
         uint32_t act = 0;
-        HIP_TRY(hipMemcpyAsync(&act, imp_.nactive, 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(&act, imp_.nactive + slot, 4, hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
         *any = act != 0;
         return MPM_OK;
@@ -515,23 +541,23 @@ int Engine::implicit_solve(const StepArgs& a) {
     int rc;
     k_imp_init<<<gb, 256, 0, s>>>(a, n);
     apply(imp_.vec[0], 0);
-    if ((rc = reduce(0))) return rc;  // kInit
+    reduce(kInit, 0);
     bool any = false;
-    if ((rc = any_active(&any))) return rc;
+    if ((rc = any_active(0, &any))) return rc;
     if (any) {
         apply(imp_.vec[1], 1);
-        if ((rc = reduce(1))) return rc;  // kRho0
+        reduce(kRho0, 1);
+        // Per iteration: the direction and solution updates with their <R,R>,
+        // the operator on R with its three dots, and one per-scene reduction;
+        // the host looks at the active count every 4 iterations.
         for (int it = 0; it < maxit; ++it) {
             k_imp_vec<<<gb, 256, 0, s>>>(a, n, 0);
-            if ((rc = reduce(2))) return rc;  // kRR
+            apply(imp_.vec[1], 2);
+            reduce(kIter, it + 2);
             if ((it & 3) == 3 || it + 1 == maxit) {
-                if ((rc = any_active(&any))) return rc;
+                if ((rc = any_active(it + 2, &any))) return rc;
                 if (!any) break;
             }
-            apply(imp_.vec[1], 2);
-            if ((rc = reduce(3))) return rc;  // kRho
-            k_imp_vec<<<gb, 256, 0, s>>>(a, n, 1);
-            if ((rc = reduce(4))) return rc;  // kDen
         }
     }
     k_imp_vec<<<gb, 256, 0, s>>>(a, n, 2);  // v = X, u = u* + (X - v*)
