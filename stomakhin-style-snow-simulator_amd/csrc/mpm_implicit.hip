@@ -338,10 +338,15 @@ __device__ __forceinline__ void corotated_dP(const T F[9], const T U[9], const T
 
 constexpr int kImpThreads = 128;
 
-__global__ __launch_bounds__(kImpThreads) void k_imp_dF(StepArgs a, const float* Q) {
+// With few home blocks the particles of a block are split over `splits`
+// workgroups (each stages the block's nodes and takes every splits-th group
+// of kImpThreads particles), so a small scene is not bound by one
+// workgroup's passes over its block.
+__global__ __launch_bounds__(kImpThreads) void k_imp_dF(StepArgs a, const float* Q, uint32_t splits) {
     __shared__ float qs[3][512];  // [channel][Z][Y][X] of the 8x8x8 nodes of the block's 8 blocks
     const uint32_t nh = a.cnt->n_home;
-    for (uint32_t blk = blockIdx.x; blk < nh; blk += gridDim.x) {
+    for (uint32_t wg = blockIdx.x; wg < nh * splits; wg += gridDim.x) {
+        const uint32_t blk = wg / splits, part = wg % splits;
         const uint32_t slot = a.g.home_list[blk];
         uint32_t sid;
         int bx, by, bz;
@@ -357,7 +362,7 @@ __global__ __launch_bounds__(kImpThreads) void k_imp_dF(StepArgs a, const float*
         __syncthreads();
         const uint32_t off = a.g.blkoff[slot], cnt = a.g.blkcnt[slot];
         const bool implicit = sc.beta > 0.0f;
-        for (uint32_t j = off + threadIdx.x; j < off + cnt; j += kImpThreads) {
+        for (uint32_t j = off + part * kImpThreads + threadIdx.x; j < off + cnt; j += splits * kImpThreads) {
             const uint32_t i = at(a.bin.sidx, j);
             float G[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
             if (implicit) {
@@ -526,8 +531,11 @@ int Engine::implicit_solve(const StepArgs& a) {
     // solve (cleared once here, so the loop has no memory-set nodes)
     HIP_TRY(hipMemsetAsync(imp_.nactive, 0, (kCRHist + 1) * 4, s));
     auto reduce = [&](int mode, int slot) { k_imp_reduce<<<gs, 256, 0, s>>>(a, nsc, mode, slot); };
+    // k_imp_dF: split blocks over workgroups while there are fewer than
+    // about 4 workgroups per CU (at most 4 ways)
+    const uint32_t splits = std::max(1u, std::min(4u, (uint32_t)(cus_ * 4) / std::max(1u, c.n_home)));
     auto apply = [&](const float* q, int mode) {
-        k_imp_dF<<<gw, kImpThreads, 0, s>>>(a, q);
+        k_imp_dF<<<gw, kImpThreads, 0, s>>>(a, q, splits);
         launch_p2g_force(a, launch_, s);
         k_imp_apply<<<gb, 256, 0, s>>>(a, n, mode);
     };

@@ -1801,7 +1801,12 @@ inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
         k_p2g<32, false><<<p2g_grid(c), 64, 0, s>>>(a);
 }
 void launch_p2g_force(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
-    k_p2g<32, true><<<p2g_grid(c), 64, 0, s>>>(a);
+    if (c.p2g_chunk == 16)
+        k_p2g<16, true><<<p2g_grid(c), 64, 0, s>>>(a);
+    else if (c.p2g_chunk == 64)
+        k_p2g<64, true><<<p2g_grid(c), 64, 0, s>>>(a);
+    else
+        k_p2g<32, true><<<p2g_grid(c), 64, 0, s>>>(a);
 }
 
 // G2P with or without the position projection (a branch per particle only
