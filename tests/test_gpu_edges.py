@@ -5,7 +5,10 @@
     B-spline vanishes at distance 2), so the step must still match;
   * a one-particle scene (a single-cell home block, every other node empty);
   * 200 particles in one cell (a cell spanning several P2G staging chunks);
-  * an engine with no resident scene (a step is a no-op).
+  * an engine with no resident scene (a step is a no-op);
+  * the semi-implicit solve with no unknowns (every node held by a
+    collider: no iteration, the explicit step bit for bit) and with an
+    iteration cap of 1.
 Bound: the 1e-4 parity metric of mpmref.rel_errors after 10 steps."""
 import numpy as np
 import pytest
@@ -71,3 +74,52 @@ def test_step_without_scenes_is_a_noop():
         sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
         eng.remove_scene(sid)
         eng.step(2)
+
+
+def _implicit(x, v, colliders, maxit=200):
+    prm = snowmpm.params(4e-4, 0.02, *snowmpm.lame(1e6, 0.2), 10.0, 0.025, 0.0075, 0.95, beta=1.0,
+                         cr_tolerance=1e-6, cr_max_iterations=maxit)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return dict(x=x, v=np.ascontiguousarray(v, dtype=np.float64), m=np.full(len(x), 4e-4), params=prm,
+                colliders=colliders)
+
+
+def test_implicit_scene_with_every_node_held_by_a_collider():
+    """A block of snow wholly inside a sticky half-space: every node with mass
+    is changed by the collider, so the solve has no unknowns and takes no
+    iteration, and the step is the explicit one bit for bit (the block stops
+    in place: velocities 0 and positions unchanged to FP32 rounding, the
+    tolerances of tests/test_gpu_contact.py)."""
+    g = np.stack(np.meshgrid(*[np.arange(6)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * 0.01 + np.array([0.2, -0.3, 0.2])
+    out = {}
+    for beta in (1.0, 0.0):
+        sc = _implicit(x, np.tile([0.0, -1.0, 0.0], (len(x), 1)), [snowmpm.make_plane((0, 0, 0), (0, 1, 0), 1e6)])
+        sc["params"].beta = beta
+        with snowmpm.Engine(capacity=len(x) + 64) as eng:
+            sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+            eng.step(1)
+            out[beta] = (eng.cr_stats(sid)["iterations"], eng.download(sid))
+            assert eng.status(sid)[0] == 0
+    assert out[1.0][0] == 0
+    for f in ("x", "v", "Fe"):
+        assert np.array_equal(out[1.0][1][f], out[0.0][1][f]), f
+    got = out[1.0][1]
+    assert np.abs(got["v"]).max() < 6e-6 and np.abs(got["x"] - x).max() < 2e-7
+
+
+def test_implicit_iteration_cap():
+    """cr_max_iterations = 1: the solve stops after one iteration, reports it,
+    and leaves a finite state; the residual history has both ends."""
+    g = np.stack(np.meshgrid(*[np.arange(8)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = (g + 0.5) * 0.01 + np.array([0.1, 0.012, 0.1])
+    sc = _implicit(x, np.tile([0.3, -2.0, 0.1], (len(x), 1)), [snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.5)],
+                   maxit=1)
+    with snowmpm.Engine(capacity=len(x) + 64) as eng:
+        sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
+        eng.step(3)
+        st = eng.cr_stats(sid)
+        got = eng.download(sid)
+        assert eng.status(sid)[0] == 0
+    assert st["iterations"] == 1 and len(st["residual"]) == 2
+    assert np.all(np.isfinite(got["v"])) and np.all(np.isfinite(got["x"]))
