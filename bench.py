@@ -291,7 +291,7 @@ def semi_implicit(seed, no_cpu, steps=20, warmup=2, beta=1.0, tol=1e-6, E=1e7, d
 
 
 def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(10, 25, 50, 100, 200),
-                       requests_per_rate=64, batched_only=False):
+                       trace_s=2.5, min_requests=64, batched_only=False):
     """Batched form (256 uneven scenes in one engine, graph-replayed step,
     autotuned launch shapes) and served form (Poisson arrivals, continuous
     batching) against the plain per-scene GPU step one scene at a time.
@@ -343,16 +343,26 @@ def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(1
     cap = 16_000_000
     eng = snowmpm.Engine(capacity=cap)
     for lam in rates:
-        gaps = rng.exponential(1.0 / lam, size=requests_per_rate)
+        # about trace_s seconds of arrivals (at least min_requests requests)
+        nreq = max(min_requests, int(round(lam * trace_s)))
+        gaps = rng.exponential(1.0 / lam, size=nreq)
         arr = np.cumsum(gaps) - gaps[0]
         reqs = []
-        for k in range(requests_per_rate):
+        for k in range(nreq):
             s = scenes[int(rng.integers(0, n_scenes))]
             reqs.append(dict(s, arrival_s=float(arr[k]), steps=int(rng.integers(100, 2001))))
         r = eng.serve(reqs, mode=0, max_chunk=25)
-        served.append({"rate_per_s": lam, "requests": requests_per_rate, "scenes_per_s": r["scenes_per_s"],
-                       "p50_s": r["p50_s"], "p99_s": r["p99_s"], "mean_s": r["mean_s"],
-                       "max_resident": r["max_resident"], "wall_s": r["wall_s"]})
+        # steady state: completions between the arrival of the request a
+        # quarter into the trace and the last arrival (no ramp-up, no drain)
+        t1, t2 = arr[nreq // 4], arr[-1]
+        done = np.array(r["done_s"])
+        steady = float(((done >= t1) & (done <= t2)).sum() / (t2 - t1))
+        served.append({"rate_per_s": lam, "requests": nreq, "scenes_per_s": r["scenes_per_s"],
+                       "steady_scenes_per_s": steady, "p50_s": r["p50_s"], "p99_s": r["p99_s"],
+                       "mean_s": r["mean_s"], "max_resident": r["max_resident"], "wall_s": r["wall_s"],
+                       "chunks": r["chunks"], "steps_per_chunk": r["chunk_steps"] / max(r["chunks"], 1),
+                       "time_step_s": r["time_step_s"], "time_admit_s": r["time_admit_s"],
+                       "time_retire_s": r["time_retire_s"]})
     eng.close()
     # plain serving at saturation (all requests queued at t=0), sampled
     eng = snowmpm.Engine(capacity=int(ns.max()) + 4096)

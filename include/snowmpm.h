@@ -243,6 +243,13 @@ typedef struct {
     int max_resident;
     int failed;              /* requests retired with a device status */
     double max_resident_bytes; /* peak sum of the resident requests' footprints */
+    /* where the loop's wall time went (mode 0): stepping (graph replays and
+     * the set-up of newly admitted scenes, to the end of each chunk),
+     * admitting (uploads) and retiring (status reads, downloads, removals);
+     * chunks and the steps they took together */
+    double time_step_s, time_admit_s, time_retire_s;
+    int chunks;
+    int chunk_steps;
 } mpm_serve_stats;
 
 /* Device bytes request q would hold in engine e: n particle slots (both

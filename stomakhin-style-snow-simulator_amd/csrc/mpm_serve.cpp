@@ -264,8 +264,8 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
     }
     const Footprint pool = pools();
     int rc;
-    double psteps = 0, max_bytes = 0;
-    int completed = 0, failed = 0, max_res = 0;
+    double psteps = 0, max_bytes = 0, t_step = 0, t_admit = 0, t_retire = 0;
+    int completed = 0, failed = 0, max_res = 0, chunks = 0, chunk_steps = 0;
     // Any error other than a scene's own status ends the call with the engine
     // emptied, so the caller can serve again.
     auto fail = [&](int code) {
@@ -334,6 +334,7 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
         };
         while (completed + failed < nreq) {
             const double tn = now();
+            double ta = tn;
             while (next < (size_t)nreq && R[order[next]].arrival_s <= tn) {
                 const int r = order[next];
                 mpm_request& q = R[r];
@@ -363,6 +364,8 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
                 max_bytes = std::max(max_bytes, (double)res.total());
                 ++next;
             }
+            const double ts = now();
+            t_admit += ts - ta;
             if (act.empty()) {
                 if (next < (size_t)nreq) wait_until(R[order[next]].arrival_s);
                 continue;
@@ -372,6 +375,10 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
             for (const auto& a : act) K = std::min(K, a.left);
             rc = step_graph(K);
             if (rc && rc != MPM_ERR_DEVICE) return fail(rc);
+            const double tr = now();
+            t_step += tr - ts;
+            ++chunks;
+            chunk_steps += K;
             pend = 0;
             size_t res_n = 0;
             for (const auto& a : act) res_n += R[a.req].n;
@@ -404,6 +411,7 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
                 ++completed;
                 retire(i);
             }
+            t_retire += now() - tr;
         }
         if ((rc = clear())) return fail(rc);
     }
@@ -412,6 +420,11 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
     st->failed = failed;
     st->max_resident = max_res;
     st->max_resident_bytes = max_bytes;
+    st->time_step_s = t_step;
+    st->time_admit_s = t_admit;
+    st->time_retire_s = t_retire;
+    st->chunks = chunks;
+    st->chunk_steps = chunk_steps;
     st->particle_steps = psteps;
     st->scenes_per_s = st->wall_s > 0 ? completed / st->wall_s : 0.0;
     std::vector<double> lat;

@@ -83,7 +83,8 @@ class ServeStats(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("scenes_per_s", C.c_double), ("p50_s", C.c_double),
                 ("p99_s", C.c_double), ("mean_s", C.c_double), ("max_s", C.c_double),
                 ("particle_steps", C.c_double), ("completed", C.c_int), ("max_resident", C.c_int),
-                ("failed", C.c_int), ("max_resident_bytes", C.c_double)]
+                ("failed", C.c_int), ("max_resident_bytes", C.c_double), ("time_step_s", C.c_double),
+                ("time_admit_s", C.c_double), ("time_retire_s", C.c_double), ("chunks", C.c_int), ("chunk_steps", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
