@@ -318,6 +318,26 @@ def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(1
     out["batched_scenes_per_s"] = n_scenes / wall
     out["batched_ms_per_step"] = wall * 1e3 / steps
     out["batched_particle_steps_per_s"] = total * steps / wall
+    # the same scenes over two engines (alternating by size), stepped
+    # together on two streams (mpm_step_graph_multi)
+    halves = [np.argsort(ns)[k::2] for k in range(2)]
+    engs = []
+    for h in halves:
+        e = snowmpm.Engine(capacity=int(ns[h].sum()) + 4096)
+        for i in h:
+            s = scenes[i]
+            e.add_scene(s["params"], s["colliders"], s["x"], s["v"], s["m"])
+        e.setup()
+        e.autotune(2)
+        engs.append(e)
+    snowmpm.step_graph_multi(engs, 5)
+    t0 = time.perf_counter()
+    snowmpm.step_graph_multi(engs, steps)
+    wall2 = time.perf_counter() - t0
+    for e in engs:
+        e.close()
+    out["two_engines_scenes_per_s"] = n_scenes / wall2
+    out["two_engines_ms_per_step"] = wall2 * 1e3 / steps
     if batched_only:
         return out
     # plain: one scene at a time, plain launches (sampled)

@@ -178,6 +178,17 @@ int mpm_probe_issue(int device, int waves_per_simd, int valu_per_mfma, int itera
  * Status is checked once at the end. */
 int mpm_step_graph(mpm_engine* e, int nsteps);
 
+/* mpm_step_graph for n engines of one device (1 <= n <= 64), stepped
+ * together: each step launches every engine's graph on that engine's own
+ * stream, so their kernels can share the GPU (a memory-bound stage of one
+ * engine beside a compute-bound stage of another).  Engines with a
+ * semi-implicit scene take their host-driven steps meanwhile.  Every engine
+ * is synchronised and checked before returning; the first error is returned,
+ * its message prefixed with "engine <index>: ".  Each scene's results are
+ * those of mpm_step_graph on its own engine.  No counterpart in the
+ * reference (its solver steps one particle set). */
+int mpm_step_graph_multi(mpm_engine* const* engines, int n, int nsteps);
+
 /* Chooses the tile sizes (P2G staging chunk 16/32/64, G2P workgroup size
  * 64/128/256), then the persistent grid sizes of the step kernels, then the
  * re-sort period among 16, 8, 4 and 2 (16 steps per candidate), by timing

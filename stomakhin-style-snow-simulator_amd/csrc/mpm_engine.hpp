@@ -62,6 +62,14 @@ class Engine {
     int setup();
     int step(int nsteps, mpm_timings* t);
     int step_graph(int nsteps);
+    // step_graph in three parts, so several engines can interleave their
+    // launches on their own streams: graph_begin sets up new scenes (or
+    // reports that a semi-implicit scene makes the step host-driven),
+    // graph_enqueue launches one step without waiting, graph_finish waits and
+    // checks the scenes' status.
+    int graph_begin(bool* host_driven);
+    int graph_enqueue();
+    int graph_finish(int nsteps);
     int autotune(int steps_per_config, int chosen[4]);
     int permute_period() const { return cfg_.permute_period; }
     void tile_sizes(int* chunk, int* threads) const {

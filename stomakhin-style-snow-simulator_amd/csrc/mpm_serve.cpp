@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -39,24 +40,39 @@ int Engine::ensure_graph(int permute) {
     return MPM_OK;
 }
 
+int Engine::graph_begin(bool* host_driven) {
+    *host_driven = implicit_active();  // the solve is host-driven: no graph
+    return *host_driven ? MPM_OK : setup();
+}
+
+int Engine::graph_enqueue() {
+    const int permute = take_permute() ? 1 : 0;
+    const int rc = ensure_graph(permute);
+    if (rc) return rc;
+    HIP_TRY(hipGraphLaunch(gexec_[(mov_ * 2 + con_) * 2 + permute], stream_));
+    advance(permute);
+    return MPM_OK;
+}
+
+int Engine::graph_finish(int nsteps) {
+    HIP_TRY(hipStreamSynchronize(stream_));
+    if (nsteps > 0)
+        for (auto& sh : scenes_) sh.pending_drop = false;
+    return check_status();
+}
+
 int Engine::step_graph(int nsteps) {
     if (nsteps < 0) {
         set_error("nsteps < 0");
         return MPM_ERR_ARG;
     }
-    if (implicit_active()) return step(nsteps, nullptr);  // the solve is host-driven: no graph
-    int rc = setup();
+    bool host_driven = false;
+    int rc = graph_begin(&host_driven);
     if (rc) return rc;
-    for (int s = 0; s < nsteps; ++s) {
-        const int permute = take_permute() ? 1 : 0;
-        if ((rc = ensure_graph(permute))) return rc;
-        HIP_TRY(hipGraphLaunch(gexec_[(mov_ * 2 + con_) * 2 + permute], stream_));
-        advance(permute);
-    }
-    HIP_TRY(hipStreamSynchronize(stream_));
-    if (nsteps > 0)
-        for (auto& sh : scenes_) sh.pending_drop = false;
-    return check_status();
+    if (host_driven) return step(nsteps, nullptr);
+    for (int s = 0; s < nsteps; ++s)
+        if ((rc = graph_enqueue())) return rc;
+    return graph_finish(nsteps);
 }
 
 int Engine::autotune(int spc, int chosen[4]) {
@@ -452,6 +468,59 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
 extern "C" {
 
 int mpm_step_graph(mpm_engine* e, int nsteps) { return e ? e->eng.step_graph(nsteps) : MPM_ERR_ARG; }
+
+int mpm_step_graph_multi(mpm_engine* const* es, int n, int nsteps) {
+    using snowmpm::set_error;
+    if (!es || n < 1 || n > 64 || nsteps < 0) {
+        set_error("mpm_step_graph_multi: need 1..64 engines and nsteps >= 0");
+        return MPM_ERR_ARG;
+    }
+    for (int k = 0; k < n; ++k) {
+        if (!es[k] || es[k]->eng.device() != es[0]->eng.device()) {
+            set_error("mpm_step_graph_multi: null engine, or engines on different devices");
+            return MPM_ERR_ARG;
+        }
+        for (int j = 0; j < k; ++j)
+            if (es[j] == es[k]) {
+                set_error("mpm_step_graph_multi: an engine is listed twice");
+                return MPM_ERR_ARG;
+            }
+    }
+    // The first error is returned (its message prefixed with the engine's
+    // index); every engine that began is still synchronised and checked.
+    int first = MPM_OK;
+    std::string msg;
+    auto note = [&](int k, int rc) {
+        if (rc && !first) {
+            first = rc;
+            msg = "engine " + std::to_string(k) + ": " + snowmpm::g_last_error;
+        }
+    };
+    std::vector<char> host(n, 0), ok(n, 0);
+    for (int k = 0; k < n; ++k) {
+        bool hd = false;
+        const int rc = es[k]->eng.graph_begin(&hd);
+        note(k, rc);
+        host[k] = hd;
+        ok[k] = rc == MPM_OK;
+    }
+    // one step of every graph engine in turn, each on its own stream
+    for (int s = 0; s < nsteps; ++s)
+        for (int k = 0; k < n; ++k)
+            if (ok[k] && !host[k]) {
+                const int rc = es[k]->eng.graph_enqueue();
+                note(k, rc);
+                if (rc) ok[k] = 0;
+            }
+    // engines with a semi-implicit scene run their host-driven steps while
+    // the graph engines' launches proceed on their streams
+    for (int k = 0; k < n; ++k)
+        if (ok[k] && host[k]) note(k, es[k]->eng.step(nsteps, nullptr));
+    for (int k = 0; k < n; ++k)
+        if (!host[k]) note(k, es[k]->eng.graph_finish(nsteps));
+    if (first) set_error(msg);
+    return first;
+}
 
 int mpm_autotune(mpm_engine* e, int steps_per_config, int chosen[4]) {
     return e ? e->eng.autotune(steps_per_config, chosen) : MPM_ERR_ARG;

@@ -115,6 +115,7 @@ _SIGS = {
     "mpm_request_footprint": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_size_t),
                                         C.POINTER(C.c_size_t)]),
     "mpm_step_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "mpm_step_graph_multi": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.c_int]),
     "mpm_autotune": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "mpm_permute_period": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "mpm_tile_sizes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -247,6 +248,14 @@ def export_frame(directory, frame_no, x):
     """export_particles (export.cpp:10-22) into directory/frame_%06d.csv."""
     x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
     _check(lib.mpm_export_frame(os.fsencode(directory), int(frame_no), x.shape[0], _dptr(x)))
+
+
+def step_graph_multi(engines, nsteps=1):
+    """step_graph() of several engines together, each launching on its own
+    stream so their kernels share the GPU; the scenes' results are those of
+    step_graph() on each engine alone."""
+    hs = (C.c_void_p * len(engines))(*[e.h for e in engines])
+    return _check(lib.mpm_step_graph_multi(hs, len(engines), int(nsteps)))
 
 
 def probe_issue(waves_per_simd, valu_per_mfma, iterations=4096, device=0):

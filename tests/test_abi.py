@@ -83,3 +83,14 @@ def test_probe_rejects_bad_arguments_without_a_gpu():
     assert lib.mpm_probe_issue(0, 1, 3, 16, out) == snowmpm.ERR_ARG      # K not in 1, 2, 4, 8, 16
     assert lib.mpm_probe_issue(0, 1, 4, 0, out) == snowmpm.ERR_ARG       # no iterations
     assert lib.mpm_probe_issue(0, 1, 4, 16, None) == snowmpm.ERR_ARG
+
+
+def test_multi_step_rejects_bad_arguments_without_a_gpu():
+    lib = ctypes.CDLL(snowmpm._LIB)
+    lib.mpm_step_graph_multi.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int]
+    none = (ctypes.c_void_p * 2)(None, None)
+    assert lib.mpm_step_graph_multi(None, 1, 1) == snowmpm.ERR_ARG         # no list
+    assert lib.mpm_step_graph_multi(none, 0, 1) == snowmpm.ERR_ARG         # no engines
+    assert lib.mpm_step_graph_multi(none, 65, 1) == snowmpm.ERR_ARG        # too many
+    assert lib.mpm_step_graph_multi(none, 2, 1) == snowmpm.ERR_ARG         # null engines
+    assert "engine" in snowmpm.last_error()

@@ -177,3 +177,54 @@ def test_tile_sizes_do_not_change_bits():
         for o in out[1:]:
             for k in ("x", "v", "Fe", "Jp"):
                 assert np.array_equal(out[0][k], o[k]), k
+
+
+def test_multi_engine_steps_equal_alone():
+    """Scenes split over two engines and stepped together
+    (mpm_step_graph_multi, each engine on its own stream), beside a third
+    engine holding a semi-implicit scene (host-driven steps in the same
+    call): every scene equals its solo run bit for bit."""
+    scs = _scenes(4, seed=9)
+    imp = snowmpm.generate_scene(0, seed=4, scale=0.2)
+    imp["params"].beta = 1.0
+    imp["params"].cr_tolerance = 1e-6
+    groups = [scs[0::2], scs[1::2], [imp]]
+    engines, sids = [], []
+    try:
+        for g in groups:
+            e = snowmpm.Engine(capacity=sum(len(s["x"]) for s in g) + 1024)
+            engines.append(e)
+            sids.append([e.add_scene(s["params"], s["colliders"], s["x"], s["v"], s["m"]) for s in g])
+        snowmpm.step_graph_multi(engines, 6)
+        snowmpm.step_graph_multi(engines[:2], 3)
+        got = [[e.download(sid) for sid in ss] for e, ss in zip(engines, sids)]
+        assert engines[2].cr_stats(sids[2][0])["iterations"] > 0
+    finally:
+        for e in engines:
+            e.close()
+    for g, gg, nsteps in zip(groups, got, (9, 9, 6)):
+        for s, b in zip(g, gg):
+            a = mpmref.run_gpu(s, nsteps)
+            for k in ("x", "v", "Fe", "Jp"):
+                assert np.array_equal(a[k], b[k]), k
+
+
+def test_multi_engine_reports_the_failing_engine():
+    """The second engine's scene has a lone particle 1 m beside its body
+    moving at 2e8 m/s, which leaves the block-coordinate range in its first
+    step (as in the served-failure test): the call returns MPM_ERR_DEVICE
+    naming engine 1, and the first engine's scene has still taken its
+    steps."""
+    good = _scenes(1, seed=2)[0]
+    bad = dict(_scenes(1, seed=3)[0])
+    bad["x"], bad["v"] = bad["x"].copy(), bad["v"].copy()
+    bad["x"][3] = (bad["x"][:, 0].max() + 1.0, bad["x"][3, 1], bad["x"][3, 2])
+    bad["v"][3] = (2.0e8, 0.0, 0.0)
+    with snowmpm.Engine(capacity=len(good["x"]) + 64) as e0, snowmpm.Engine(capacity=len(bad["x"]) + 64) as e1:
+        s0 = e0.add_scene(good["params"], good["colliders"], good["x"], good["v"], good["m"])
+        e1.add_scene(bad["params"], bad["colliders"], bad["x"], bad["v"], bad["m"])
+        with pytest.raises(snowmpm.MpmError) as err:
+            snowmpm.step_graph_multi([e0, e1], 2)
+        assert err.value.code == snowmpm.ERR_DEVICE
+        assert "engine 1" in str(err.value)
+        assert e0.steps(s0) == 2
