@@ -228,3 +228,22 @@ def test_multi_engine_reports_the_failing_engine():
         assert err.value.code == snowmpm.ERR_DEVICE
         assert "engine 1" in str(err.value)
         assert e0.steps(s0) == 2
+
+
+def test_served_mix_with_a_semi_implicit_request():
+    """A served trace in which one request has beta = 1: chunks that hold it
+    take the host-driven steps of the solve, and every request, implicit or
+    explicit, ends with the bits of its solo run."""
+    scs = _scenes(3, seed=13)
+    imp = snowmpm.generate_scene(0, seed=6, scale=0.2)
+    imp["params"].beta, imp["params"].cr_tolerance = 1.0, 1e-6
+    reqs = [dict(scs[0], arrival_s=0.0, steps=9), dict(imp, arrival_s=0.0, steps=6),
+            dict(scs[1], arrival_s=0.002, steps=5), dict(scs[2], arrival_s=0.004, steps=8)]
+    cap = sum(len(q["x"]) for q in reqs) + 1024
+    with snowmpm.Engine(capacity=cap) as eng:
+        res = eng.serve(reqs, mode=0, max_chunk=4, keep_outputs=True)
+    assert res["completed"] == len(reqs) and res["failed"] == 0
+    for q, (ox, ov) in zip(reqs, res["outputs"]):
+        alone = mpmref.run_gpu(q, q["steps"])
+        assert np.array_equal(alone["x"], ox)
+        assert np.array_equal(alone["v"], ov)
