@@ -24,6 +24,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serv
                        [--no-implicit]
        python bench.py --batched-only --steps K   (the batched form alone, for profiling)
        python bench.py --issue-probe               (MFMA/VALU issue rates behind the P2G bound)
+       python bench.py --stream-probe              (HBM rate of a copy and of G2P's particle streams)
+       python bench.py --config-baselines          (configs 0, 1, 3: plain GPU step beside the FP64 oracle)
 """
 import argparse
 import json
@@ -161,6 +163,71 @@ def cpu_baseline(sc, seed, scale=1.0, steps=3):
     return out
 
 
+def config_baselines(seed, no_cpu):
+    """The other configurations' plain-method timings beside the FP64 oracle's,
+    as the measurement plan lays them out: config 0 (the 64,000-particle cube,
+    all 200 steps), config 1 (the two tori, 10 steps of the 2000), config 3
+    (16 of the 256 batched scenes at 50 steps each, extrapolated to the
+    1000-step scenes/s of all 256).  The GPU figures are the plain one-scene
+    step (config 3: per scene, one at a time, the same 16 scenes)."""
+    import numpy as np
+    import snowmpm
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    info = _host_info()
+    threads = min(info["affinity"], int(info["OMP_NUM_THREADS"] or info["affinity"]))
+    out = {}
+
+    def gpu_rate(sc, steps):
+        n = sc["x"].shape[0]
+        with snowmpm.Engine(capacity=n + 1024) as eng:
+            eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"], E=sc.get("E"))
+            eng.setup()
+            eng.step(2)
+            t0 = time.perf_counter()
+            eng.step(steps)
+            return n * steps / (time.perf_counter() - t0)
+
+    def cpu_rate(sc, steps):
+        import mpmref  # the oracle driver, used here only as the timed CPU baseline
+        import oracle as orc
+        prm = sc["params"]
+        mu0 = la0 = None
+        if sc.get("E") is not None:
+            mu0, la0 = snowmpm.lame(sc["E"], prm.nu)
+        st = orc.State(sc["x"], sc["v"], sc["m"], mu0=mu0, lambda0=la0)
+        st.setup(prm.dx)
+        t0 = time.perf_counter()
+        st.run(mpmref.orc_colliders(sc["colliders"]), mpmref.orc_params(prm), steps, nthreads=threads)
+        return sc["x"].shape[0] * steps / (time.perf_counter() - t0)
+
+    for cfg, steps, label in ((0, 200, "cube 0.4 m, 64,000 particles, all 200 steps"),
+                              (1, 10, "two tori, 1,365,824 particles, 10 of the 2000 steps")):
+        sc = snowmpm.generate_scene(cfg, seed=seed)
+        row = {"workload": label, "gpu_particle_steps_per_s": gpu_rate(sc, 200 if cfg == 0 else 100)}
+        if not no_cpu:
+            row["cpu_particle_steps_per_s"] = cpu_rate(sc, steps)
+            row["gpu_over_cpu"] = row["gpu_particle_steps_per_s"] / row["cpu_particle_steps_per_s"]
+        out[f"config{cfg}"] = row
+    # config 3: 16 of the 256 scenes, 50 steps each, extrapolated to 1000 steps
+    idx = np.linspace(0, 255, 16).astype(int)
+    scenes = [snowmpm.generate_scene(3, seed=seed, index=int(i)) for i in idx]
+    ns = np.array([s["x"].shape[0] for s in scenes])
+    all_ns = np.array([snowmpm.generate_scene(3, seed=seed, index=i)["x"].shape[0] for i in range(256)])
+    scale = all_ns.mean() / ns.mean()
+    row = {"workload": "16 of the 256 batched scenes, 50 steps each, extrapolated to 1000-step scenes/s "
+                       "(scaled by mean size)"}
+    gpu_t = sum(s["x"].shape[0] * 50 / gpu_rate(s, 50) for s in scenes)
+    row["gpu_plain_scenes_per_s"] = 1.0 / (gpu_t / len(scenes) * 20 * scale)
+    if not no_cpu:
+        cpu_t = sum(s["x"].shape[0] * 50 / cpu_rate(s, 50) for s in scenes)
+        row["cpu_scenes_per_s"] = 1.0 / (cpu_t / len(scenes) * 20 * scale)
+        row["gpu_plain_over_cpu"] = row["gpu_plain_scenes_per_s"] / row["cpu_scenes_per_s"]
+    out["config3"] = row
+    out["cpu"] = {"cores": threads, "kind": "port", "note": "FP64 oracle, the reference's serial/OpenMP split"}
+    return out
+
+
 def issue_probe():
     """SIMD cycles per iteration of 4 independent FP32 MFMAs, of 4 K FP32
     FMAs and of both interleaved, at 1, 2 and 4 waves per SIMD
@@ -176,6 +243,25 @@ def issue_probe():
                          "mixed_over_sum": round(r["mixed"] / (r["mfma"] + r["valu"]), 3),
                          "mixed_over_max": round(r["mixed"] / max(r["mfma"], r["valu"]), 3),
                          "kernel_us": [round(u, 1) for u in r["us"]]})
+    return rows
+
+
+def stream_probe():
+    """HBM rate with no arithmetic (mpm_probe_stream): a 1 GiB float4 copy,
+    and G2P's particle streams on the 8M slab (19 FP32 arrays read, 22
+    written) by 512-element workgroups as G2P takes a home block, and one
+    element per thread."""
+    import snowmpm
+    names = {0: "copy 1 GiB, float4", 1: "G2P streams, 128 threads per 512 elements",
+             2: "G2P streams, one element per thread"}
+    rows = []
+    for mode in (0, 1, 2):
+        best = None
+        for _ in range(3):
+            r = snowmpm.probe_stream(mode)
+            best = r if best is None or r["us"] < best["us"] else best
+        rows.append({"mode": mode, "pattern": names[mode], "us": round(best["us"], 1), "bytes": int(best["bytes"]),
+                     "tb_per_s": round(best["tb_per_s"], 3)})
     return rows
 
 
@@ -478,6 +564,11 @@ def main():
                     help="write profiles/traffic.json from two rocprofv3 counter CSVs (no GPU work)")
     ap.add_argument("--issue-probe", action="store_true",
                     help="only the MFMA/VALU issue-rate probe (one JSON line)")
+    ap.add_argument("--config-baselines", action="store_true",
+                    help="only configs 0, 1 and 3 on the plain GPU step beside the FP64 oracle (a few minutes "
+                         "of CPU work; one JSON line)")
+    ap.add_argument("--stream-probe", action="store_true",
+                    help="only the HBM streaming probe (copy and G2P's particle streams; one JSON line)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the run, time one 1 GiB device-to-device copy (traffic-counter calibration)")
     args = ap.parse_args()
@@ -494,6 +585,12 @@ def main():
 
     if args.issue_probe:
         print(json.dumps({"issue_probe": issue_probe()}))
+        return
+    if args.stream_probe:
+        print(json.dumps({"stream_probe": stream_probe()}))
+        return
+    if args.config_baselines:
+        print(json.dumps({"configs_plain": config_baselines(args.seed, args.no_cpu)}))
         return
     if args.batched_only:
         print(json.dumps({"batched": batched_and_served(args.seed, steps=args.steps, batched_only=True)}))

@@ -172,6 +172,15 @@ int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
  * iteration of the slowest wave, out[3 + mode] = kernel microseconds. */
 int mpm_probe_issue(int device, int waves_per_simd, int valu_per_mfma, int iterations, double out[6]);
 
+/* HBM streaming probe (no counterpart in the reference; a measurement for
+ * the G2P bound): on `device`, mode 0 copies 1 GiB as float4, mode 1 reads
+ * 19 and writes 22 FP32 arrays of 8,000,000 elements (G2P's particle streams
+ * on the 8M slab) with a 128-thread workgroup per 512 consecutive elements,
+ * mode 2 the same with one element per thread.  out[0] = microseconds per
+ * launch (mean of 10), out[1] = bytes read + written per launch,
+ * out[2] = TB/s. */
+int mpm_probe_stream(int device, int mode, double out[3]);
+
 /* The same steps replayed from a captured hipGraph (one graph per buffer
  * parity, re-captured only when the launch setup changes); the device-side
  * particle/block counts make one graph valid for any set of resident scenes.

@@ -1,7 +1,9 @@
-// mpm_probe.hip -- issue-rate probe behind the design's bound for P2G: how many
-// SIMD cycles a v_mfma_f32_16x16x4f32 and a wave64 FP32 FMA take, alone and
-// interleaved.  If the matrix core and the VALU issued side by side, the
-// mixed loop would cost the larger of the two alone; it costs their sum.
+// mpm_probe.hip -- measurements behind the kernels' bounds.  The issue-rate
+// probe: how many SIMD cycles a v_mfma_f32_16x16x4f32 and a wave64 FP32 FMA
+// take, alone and interleaved (if the matrix core and the VALU issued side
+// by side, the mixed loop would cost the larger of the two alone; it costs
+// their sum).  The streaming probe: the HBM rate of a plain copy and of
+// G2P's particle streams with no arithmetic.
 //
 // Every wave runs `iters` iterations of 4 independent MFMA accumulations
 // (mode 0), of 4 K FMAs over 8 independent chains (mode 1), or of both
@@ -77,7 +79,84 @@ static int probe_run(int mode, int blocks, int iters, unsigned long long* dcyc, 
     return MPM_OK;
 }
 
+// ---------------------------------------------------------------------------
+// HBM streaming probe: the rate a kernel reaches when it only moves bytes,
+// for a plain copy and for the particle streams of G2P on the 8M-particle
+// slab (19 FP32 arrays read: x v F_E J_P V0 E and the sorted index; 22
+// written: x v F_E J_P S), so a kernel's traffic can be set against what
+// its access pattern allows on this part.
+constexpr uint32_t kStreamN = 8000000u;
+constexpr int kStreamR = 19, kStreamW = 22;
+
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ a, float4* __restrict__ b, size_t n4) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n4) b[i] = a[i];
+}
+// ONE = false: a 128-thread workgroup per 512 consecutive elements (4
+// passes), as G2P takes a home block; ONE = true: one element per thread.
+template <bool ONE>
+__global__ __launch_bounds__(256) void k_stream_fields(const float* __restrict__ in, float* __restrict__ out,
+                                                       uint32_t n) {
+    const uint32_t first = ONE ? blockIdx.x * blockDim.x + threadIdx.x : blockIdx.x * 512u + threadIdx.x;
+    const uint32_t last = ONE ? first + 1u : blockIdx.x * 512u + 512u;
+    for (uint32_t j = first; j < last && j < n; j += blockDim.x) {
+        float s = 0.f;
+#pragma unroll
+        for (int e = 0; e < kStreamR; ++e) s += in[(size_t)e * n + j];
+#pragma unroll
+        for (int e = 0; e < kStreamW; ++e) out[(size_t)e * n + j] = s + (float)e;
+    }
+}
+
 }  // namespace snowmpm
+
+extern "C" int mpm_probe_stream(int device, int mode, double out[3]) {
+    using namespace snowmpm;
+    if (!out || mode < 0 || mode > 2) return MPM_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return MPM_ERR_HIP;
+    const size_t copy_bytes = (size_t)1 << 30;
+    const size_t nin = mode == 0 ? copy_bytes : (size_t)kStreamR * kStreamN * sizeof(float);
+    const size_t nout = mode == 0 ? copy_bytes : (size_t)kStreamW * kStreamN * sizeof(float);
+    const double moved = (double)(nin + nout);
+    void *din = nullptr, *dout = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MPM_OK;
+    if (hipMalloc(&din, nin) != hipSuccess || hipMalloc(&dout, nout) != hipSuccess ||
+        hipMemset(din, 0, nin) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess)
+        rc = MPM_ERR_HIP;
+    auto launch = [&]() {
+        if (mode == 0) {
+            const size_t n4 = copy_bytes / sizeof(float4);
+            k_stream_copy<<<(unsigned)((n4 + 255) / 256), 256>>>((const float4*)din, (float4*)dout, n4);
+        } else if (mode == 1) {
+            k_stream_fields<false><<<(kStreamN + 511) / 512, 128>>>((const float*)din, (float*)dout, kStreamN);
+        } else {
+            k_stream_fields<true><<<(kStreamN + 255) / 256, 256>>>((const float*)din, (float*)dout, kStreamN);
+        }
+    };
+    constexpr int kTimed = 10;
+    float ms = 0.f;
+    if (rc == MPM_OK) {
+        launch();  // warm-up
+        launch();
+        if (hipEventRecord(e0, nullptr) != hipSuccess) rc = MPM_ERR_HIP;
+        for (int k = 0; k < kTimed && rc == MPM_OK; ++k) launch();
+        if (rc == MPM_OK && (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess))
+            rc = MPM_ERR_HIP;
+    }
+    if (rc == MPM_OK) {
+        out[0] = ms * 1e3 / kTimed;
+        out[1] = moved;
+        out[2] = moved / (out[0] * 1e-6) / 1e12;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (din) (void)hipFree(din);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}
 
 extern "C" int mpm_probe_issue(int device, int waves_per_simd, int valu_per_mfma, int iterations, double out[6]) {
     using namespace snowmpm;

@@ -110,6 +110,7 @@ _SIGS = {
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
     "mpm_probe_issue": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pd]),
+    "mpm_probe_stream": (C.c_int, [C.c_int, C.c_int, _pd]),
     "mpm_scene_steps": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
     "mpm_cr_stats": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _pd, C.c_int]),
     "mpm_request_footprint": (C.c_int, [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_size_t),
@@ -265,6 +266,16 @@ def probe_issue(waves_per_simd, valu_per_mfma, iterations=4096, device=0):
     out = (C.c_double * 6)()
     _check(lib.mpm_probe_issue(int(device), int(waves_per_simd), int(valu_per_mfma), int(iterations), out))
     return dict(mfma=out[0], valu=out[1], mixed=out[2], us=(out[3], out[4], out[5]))
+
+
+def probe_stream(mode, device=0):
+    """HBM rate with no arithmetic: mode 0 a 1 GiB float4 copy, mode 1 G2P's
+    particle streams on the 8M slab (19 FP32 arrays read, 22 written) by
+    512-element workgroups, mode 2 the same one element per thread
+    (see mpm_probe_stream)."""
+    out = (C.c_double * 3)()
+    _check(lib.mpm_probe_stream(int(device), int(mode), out))
+    return dict(us=out[0], bytes=out[1], tb_per_s=out[2])
 
 
 def seed_mesh(verts, tris, spacing, device=0):
