@@ -24,6 +24,12 @@ constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr int kBlockNodes = 64;             // 4x4x4 nodes per block
 constexpr int kGridCh = 7;                  // P2G channels: m, p(3), f(3)
 constexpr int kRegion = 343;                // 7x7x7 nodes a home block's particles reach
+// A home block's P2G partial sums, node plane by node plane: each of the 7
+// planes holds its 4 xy pieces (4x4, 3x4, 4x3, 3x3 nodes) x 7 channels, each
+// piece padded to a multiple of 32 floats so that it starts on a 128-byte
+// line: 128 + 96 + 96 + 64 = 384 floats per plane.
+constexpr int kPlaneFloats = 384;
+constexpr int kPartFloats = 7 * kPlaneFloats;  // per home block
 constexpr int kVelCh = 6;                   // FP32 outputs: v(3), u(3) (u = PIC/FLIP blend)
 constexpr int kStressCh = 6;                // symmetric force matrix -V0 P F^T
 

@@ -148,10 +148,11 @@ int Engine::init(const mpm_config& cfg) {
     if ((rc = dalloc((void**)&grid_.blkoff, (size_t)H_ * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.blkcnt, (size_t)H_ * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.nb, (size_t)H_ * 8 * 4))) return rc;
-    // P2G partials: one 7x7x7x7 float region per home block; a home block
-    // holds at least one particle, so N bounds the count as well as H / 2.
+    // P2G partials: the 7 node planes of a 7x7x7-node region, 7 channels, per
+    // home block (kPartFloats); a home block holds at least one particle, so
+    // N bounds the count as well as H / 2.
     grid_.max_home = (uint32_t)std::min<size_t>(N + 64, H_ / 2);
-    if ((rc = dalloc((void**)&grid_.part, (size_t)grid_.max_home * kGridCh * kRegion * 4))) return rc;
+    if ((rc = dalloc((void**)&grid_.part, (size_t)grid_.max_home * kPartFloats * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.rev, (size_t)H_ * 8 * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.vu, (size_t)H_ * kVelCh * 64 * 4))) return rc;
     if ((rc = dalloc((void**)&grid_.home_list, (size_t)H_ * 4))) return rc;

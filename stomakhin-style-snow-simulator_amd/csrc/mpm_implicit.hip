@@ -108,12 +108,14 @@ __global__ __launch_bounds__(256) void k_imp_apply(StepArgs a, uint32_t n, int m
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int nx = ((t & 1) ? 3 : 4), ny = (((t >> 1) & 1) ? 3 : 4), nz = ((t >> 2) ? 3 : 4), nxy = nx * ny;
+            // xy piece t & 3 of a plane starts at the padded offset of the
+            // pieces before it (7 nx ny floats each, rounded up to 32)
             int xyb = 0;
-            for (int u = 0; u < (t & 3); ++u) xyb += ((u & 1) ? 3 : 4) * (((u >> 1) & 1) ? 3 : 4);
+            for (int u = 0; u < (t & 3); ++u) xyb += (kGridCh * ((u & 1) ? 3 : 4) * (((u >> 1) & 1) ? 3 : 4) + 31) / 32 * 32;
             const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)revl, t);
             if (hb != kInvalid && hb < a.g.max_home && lx < nx && ly < ny && lz < nz) {
-                const float* src = a.g.part + (size_t)hb * kGridCh * kRegion + (4 * (t >> 2) + lz) * (kGridCh * 49) +
-                                   kGridCh * xyb + ly * nx + lx;
+                const float* src = a.g.part + (size_t)hb * kPartFloats + (4 * (t >> 2) + lz) * kPlaneFloats + xyb +
+                                   ly * nx + lx;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) df[r] += src[(4 + r) * nxy];
             }

@@ -553,9 +553,11 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
 // bit 0, 3 with bit 1.  part stores a home block's sums plane by plane
-// (plane Z = 4 tz + lz of the region: 343 floats), each plane as its 4 xy
-// pieces txy = tx + 2 ty (offset 7 xy_base(txy)), channel-major, nodes
-// x-fastest.
+// (plane Z = 4 tz + lz of the region: kPlaneFloats = 384 floats), each plane
+// as its 4 xy pieces txy = tx + 2 ty (at plane_piece(txy): each piece padded
+// to a whole number of 128-byte lines), channel-major, nodes x-fastest.
+// P2G writes every plane whole, padding included, so no line is written in
+// part.
 __host__ __device__ constexpr int piece_n(int t, int axis) { return ((t >> axis) & 1) ? 3 : 4; }
 __host__ __device__ constexpr int piece_base(int t) {
     int b = 0;
@@ -569,6 +571,14 @@ __host__ __device__ constexpr int xy_base(int txy) {
     return b;
 }
 static_assert(xy_base(4) == 49, "xy pieces tile a 7x7 plane");
+// Offset of xy piece txy within a plane of partial sums (pieces of
+// 7 nx ny floats, each padded to a multiple of 32 floats).
+__host__ __device__ constexpr int plane_piece(int txy) {
+    int b = 0;
+    for (int u = 0; u < txy; ++u) b += (kGridCh * piece_n(u, 0) * piece_n(u, 1) + 31) / 32 * 32;
+    return b;
+}
+static_assert(plane_piece(4) == kPlaneFloats, "padded pieces fill a plane");
 
 // Region of a wave: [x half 0..1][channel 0..6][Y 0..6][Z mod 4][x 0..3] (x
 // half h holds x nodes 4h..4h+3).  The channel stride, 112 floats, is 16
@@ -588,22 +598,27 @@ constexpr int kP2GRecC = 16;
 // staging floats of a CHUNK: the records, the all-zero record, 2 floats of slack
 template <int CHUNK>
 constexpr int p2g_pst() { return (CHUNK + 1) * kP2GRec + 2; }
-constexpr int kP2GFlushQ = (kRegion + 63) / 64;  // plane outputs per lane
+constexpr int kP2GFlushQ = kPlaneFloats / 64;  // plane outputs per lane
+static_assert(kPlaneFloats % 64 == 0, "whole plane outputs per lane");
 
-// Plane table: element e of a plane's partial sums (order [txy][c][ly][lx],
-// see piece_n) is region index kPlaneSrc.e[e] + (Z mod 4) kP2GStrZ.
+// Plane table: element e of a plane's partial sums (piece txy at
+// plane_piece(txy), then [c][ly][lx]) is region index kPlaneSrc.e[e] +
+// (Z mod 4) kP2GStrZ.  The padding after each piece reads an x = 7 entry of
+// the region, which the deposits never change from 0, so every 128-byte
+// line of a plane is written whole.
 struct PlaneSrc {
-    uint16_t e[kRegion];
+    uint16_t e[kPlaneFloats];
 };
 constexpr PlaneSrc make_plane_src() {
     PlaneSrc t{};
+    for (int e = 0; e < kPlaneFloats; ++e) t.e[e] = (uint16_t)(kP2GStrH + 3);  // x node 7: always 0
     for (int txy = 0; txy < 4; ++txy) {
         const int nx = piece_n(txy, 0), ny = piece_n(txy, 1);
         for (int c = 0; c < kGridCh; ++c)
             for (int ly = 0; ly < ny; ++ly)
                 for (int lx = 0; lx < nx; ++lx) {
                     const int X = 4 * (txy & 1) + lx, Y = 4 * (txy >> 1) + ly;
-                    t.e[kGridCh * xy_base(txy) + c * nx * ny + ly * nx + lx] =
+                    t.e[plane_piece(txy) + c * nx * ny + ly * nx + lx] =
                         (uint16_t)((X >> 2) * kP2GStrH + c * kP2GStrCh + Y * kP2GStrY + (X & 3));
                 }
     }
@@ -745,7 +760,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
 #pragma unroll
     for (int q = 0; q < kP2GFlushQ; q += 2) {
         const int e0 = lane + 64 * q, e1 = e0 + 64;
-        fsrc[q / 2] = (e0 < kRegion ? kPlaneSrc.e[e0] : 0u) | ((e1 < kRegion ? (uint32_t)kPlaneSrc.e[e1] : 0u) << 16);
+        fsrc[q / 2] = kPlaneSrc.e[e0] | ((uint32_t)kPlaneSrc.e[e1] << 16);
     }
 
     // Chunk pipeline over this wave's blocks blk, blk + G, ...: while chunk k
@@ -810,22 +825,21 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         const float inv_dx = a.scenes[sid].inv_dx;
         const bool write = blk < a.g.max_home;
         if (!write && lane == 0) flag(a.st, sid, kStatusOverflow);
-        float* out = a.g.part + (size_t)(write ? blk : 0u) * kGridCh * kRegion;
+        float* out = a.g.part + (size_t)(write ? blk : 0u) * kPartFloats;
         // Node plane Z is complete: write it as the block's partial sums for
-        // that plane (343 consecutive floats) and clear its ring slot.
+        // that plane (384 consecutive floats, padding included) and clear
+        // its ring slot.
         int zf = 0;  // next plane to write
         auto flush_plane = [&](int Z) {
             const int slot = (Z & 3) * kP2GStrZ;
-            float* po = out + Z * (kGridCh * 49);
+            float* po = out + Z * kPlaneFloats;
 #pragma unroll
             for (int q = 0; q < kP2GFlushQ; ++q) {
                 const int e = lane + 64 * q;
-                if (e < kRegion) {
-                    float* src = reg + (int)((fsrc[q / 2] >> (16 * (q & 1))) & 0xFFFFu) + slot;
-                    const float v = *src;
-                    *src = 0.0f;
-                    if (write) po[e] = v;
-                }
+                float* src = reg + (int)((fsrc[q / 2] >> (16 * (q & 1))) & 0xFFFFu) + slot;
+                const float v = *src;
+                *src = 0.0f;
+                if (write) po[e] = v;
             }
         };
         // x-row accumulators, C_M and C_F: cells at even cx add to x nodes
@@ -988,12 +1002,12 @@ __global__ __launch_bounds__(256) void k_grid(StepArgs a) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             // node (lx, ly, lz) of piece t of home block hb: plane 4 tz + lz of
-            // its region, xy piece t & 3 (layout at xy_base)
+            // its region, xy piece t & 3 (at plane_piece)
             const int nx = piece_n(t, 0), ny = piece_n(t, 1), nz = piece_n(t, 2), nxy = nx * ny;
             const uint32_t hb = (uint32_t)__builtin_amdgcn_readlane((int)revl, t);
             if (hb != kInvalid && lx < nx && ly < ny && lz < nz) {
-                const float* src = a.g.part + (size_t)hb * kGridCh * kRegion +
-                                   (4 * (t >> 2) + lz) * (kGridCh * 49) + kGridCh * xy_base(t & 3) + ly * nx + lx;
+                const float* src = a.g.part + (size_t)hb * kPartFloats + (4 * (t >> 2) + lz) * kPlaneFloats +
+                                   plane_piece(t & 3) + ly * nx + lx;
 #pragma unroll
                 for (int c = 0; c < kGridCh; ++c) q[c] += src[c * nxy];
             }

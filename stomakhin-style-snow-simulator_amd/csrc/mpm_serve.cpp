@@ -197,7 +197,7 @@ static size_t bytes_per_particle(bool track) {
     return 2 * (nf * 4 + 8) + 4 * 4 + 4 + 36;
 }
 constexpr size_t kBytesPerBlock = 8 + 2 * 64 * 4 + 4 + 4 + 8 * 4 + 8 * 4 + kVelCh * 64 * 4 + 4 + 4;
-constexpr size_t kBytesPerHome = (size_t)kGridCh * kRegion * 4;
+constexpr size_t kBytesPerHome = (size_t)kPartFloats * 4;
 
 Footprint Engine::pools() const {
     return Footprint{cfg_.particle_capacity * bytes_per_particle(cfg_.track_fp != 0), (size_t)(H_ / 2) * kBytesPerBlock,
