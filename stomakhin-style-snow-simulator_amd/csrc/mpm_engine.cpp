@@ -83,6 +83,11 @@ int Engine::init(const mpm_config& cfg) {
     HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, cfg_.device));
     cus_ = cus;
     const size_t N = cfg_.particle_capacity;
+    // Home blocks take sorted ranges that start on 32-particle boundaries
+    // when the holes fit (k_blocks), so a re-sort leaves holes in the
+    // buffers: a quarter of the capacity is kept for them.
+    slots_ = cfg_.dense_blocks ? N : std::min<size_t>((N + N / 4 + 64 + 63) & ~(size_t)63, ((size_t)1 << 30) - 64);
+    if (slots_ < N) slots_ = N;
     launch_.grid_p = (int)std::min<size_t>((N + 255) / 256, (size_t)cus * 16);
     launch_.grid_b = cus * 8;
     launch_.grid_w = cus * 8;
@@ -115,10 +120,11 @@ int Engine::init(const mpm_config& cfg) {
         PBuf& p = buf_[b];
         std::memset(&p, 0, sizeof p);
         const int nf = 3 + 3 + 9 + 1 + (track ? 9 : 0) + 1 + 1 + 1 + 6;
+        const size_t S = slots_;
         float* base;
-        if ((rc = dalloc((void**)&base, N * sizeof(float) * nf))) return rc;
+        if ((rc = dalloc((void**)&base, S * sizeof(float) * nf))) return rc;
         float* q = base;
-        auto take = [&]() { float* r = q; q += N; return r; };
+        auto take = [&]() { float* r = q; q += S; return r; };
         for (int e = 0; e < 3; ++e) p.x[e] = take();
         for (int e = 0; e < 3; ++e) p.v[e] = take();
         for (int e = 0; e < 9; ++e) p.F[e] = take();
@@ -129,14 +135,14 @@ int Engine::init(const mpm_config& cfg) {
         p.V0 = take();
         p.E = take();
         for (int e = 0; e < 6; ++e) p.S[e] = take();
-        p.stride = (uint32_t)N;  // components of x, v, F, Fp and S lie N elements apart
-        if ((rc = dalloc((void**)&p.pid, N * 4))) return rc;
-        if ((rc = dalloc((void**)&p.sid, N * 4))) return rc;
+        p.stride = (uint32_t)S;  // components of x, v, F, Fp and S lie slots_ elements apart
+        if ((rc = dalloc((void**)&p.pid, S * 4))) return rc;
+        if ((rc = dalloc((void**)&p.sid, S * 4))) return rc;
     }
-    if ((rc = dalloc((void**)&bin_.pkey, N * 4))) return rc;
-    if ((rc = dalloc((void**)&bin_.prank, N * 4))) return rc;
-    if ((rc = dalloc((void**)&bin_.sidx, N * 4))) return rc;
-    if ((rc = dalloc((void**)&bin_.spid, N * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.pkey, slots_ * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.prank, slots_ * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.sidx, slots_ * 4))) return rc;
+    if ((rc = dalloc((void**)&bin_.spid, slots_ * 4))) return rc;
     if ((rc = dalloc((void**)&fb_.pos, N * 4))) return rc;
     if ((rc = dalloc((void**)&fb_.L, N * 36))) return rc;
 
@@ -193,6 +199,8 @@ StepArgs Engine::args(uint32_t pass_flag, int permute) const {
     a.kappa = (float)cfg_.kappa;
     a.permute = permute;
     a.project = any_project_ ? 1 : 0;
+    a.slots = (uint32_t)slots_;
+    a.align_blocks = cfg_.dense_blocks ? 0 : 1;
     if (implicit_active()) a.imp = imp_;
     return a;
 }
@@ -309,9 +317,20 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     uint32_t n0;
     int rc = device_count(&n0);
     if (rc) return rc;
-    if ((size_t)n0 + n > cfg_.particle_capacity) {
-        set_error("particle capacity exceeded: " + std::to_string(n0) + " + " + std::to_string(n) + " > " +
+    size_t live = 0;
+    for (const auto& sh : scenes_)
+        if (sh.used) live += sh.n;
+    if (live + n > cfg_.particle_capacity) {
+        set_error("particle capacity exceeded: " + std::to_string(live) + " + " + std::to_string(n) + " > " +
                   std::to_string(cfg_.particle_capacity));
+        return MPM_ERR_CAPACITY;
+    }
+    // The new particles go after the buffer's last slot in use (holes and
+    // removed scenes included until the next re-sort compacts them).
+    if ((size_t)n0 + n > slots_) {
+        set_error("particle slots exceeded: " + std::to_string(n0) + " in use (removed scenes and block-aligned "
+                  "holes included until the next step) + " + std::to_string(n) + " > " + std::to_string(slots_) +
+                  "; step once to compact");
         return MPM_ERR_CAPACITY;
     }
     // The caller's double arrays go to the GPU as they are; a kernel converts

@@ -132,6 +132,10 @@ class Engine {
     hipStream_t stream_ = nullptr;
     LaunchCfg launch_{};
     uint32_t H_ = 0;
+    // Particle slots per buffer (and sorted positions): the capacity plus room
+    // for the holes that start every home block's range on a 32-particle
+    // (128-byte) boundary.
+    size_t slots_ = 0;
     PBuf buf_[2]{};
     // Current particle state: moving fields in buf_[mov_], constant fields in
     // buf_[con_].  Every step writes the moving fields to the other buffer;

@@ -480,7 +480,7 @@ inline unsigned blocks_for(uint32_t n, unsigned per, unsigned cap) {
 
 int Engine::implicit_alloc() {
     if (imp_.gm) return MPM_OK;
-    const size_t N = cfg_.particle_capacity, H = H_;
+    const size_t N = slots_, H = H_;  // G is indexed like the particle buffers
     int rc;
     if ((rc = dalloc((void**)&imp_.gm, H * 64 * 4))) return rc;
     for (int k = 0; k < kCRVecs; ++k)

@@ -101,7 +101,7 @@ struct SceneCache {
 __device__ __forceinline__ BinItem bin_item(const StepArgs& a, const SceneCache& sc_c, uint32_t i, uint32_t n,
                                             uint32_t sid, float x0, float x1, float x2) {
     BinItem it{kEmptyKey, 0u, sid, false};
-    if (i >= n) return it;
+    if (i >= n || sid >= (uint32_t)kMaxScenes) return it;  // past the end, or a hole a re-sort left
     int flags;
     float inv_dx;
     if (sid == sc_c.sid) {
@@ -176,6 +176,7 @@ __global__ __launch_bounds__(256) void k_bin(StepArgs a) {
     load(wbase);
     SceneCache scc;
     scc.sid = (uint32_t)__builtin_amdgcn_readfirstlane((int)nsid[0]);
+    if (scc.sid >= (uint32_t)kMaxScenes) scc.sid = 0u;  // a hole: any scene serves as the cache
     scc.flags = a.scenes[scc.sid].flags;
     scc.inv_dx = a.scenes[scc.sid].inv_dx;
     for (; wbase < n; wbase += stride) {
@@ -283,6 +284,17 @@ __global__ __launch_bounds__(1024) void k_list(StepArgs a) {
     }
 }
 
+// Sorted ranges of home blocks start on 32-particle (128-byte) boundaries
+// when the holes fit: at most 31 per home block beyond the particles, which
+// the buffer's slots in use (n_particles) bound.  Then every range G2P,
+// k_scatter and k_cellsort write covers whole lines of each field, not parts
+// of lines that a neighbouring range's workgroup also writes; after a re-sort
+// the buffers keep that layout, with holes (scene id kInvalid) between the
+// blocks.  The decision is per pass and the same in every kernel that asks.
+__device__ __forceinline__ bool block_ranges_aligned(const StepArgs& a, uint32_t nh) {
+    return a.align_blocks && (uint64_t)a.cnt->n_particles + 31ull * nh <= (uint64_t)a.slots;
+}
+
 // Per home block (one wave): exclusive scan of the 64 cell counts, particle
 // range, and the 8 blocks its nodes live in (the block itself + {0,1}^3).
 // A workgroup of 16 waves takes one particle range for its 16 blocks.  The
@@ -295,6 +307,7 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const uint32_t nh = a.cnt->n_home;
     const uint32_t stride = gridDim.x * 16;
+    const bool pad = block_ranges_aligned(a, nh);
     // The next block's slot, cell counts and key are loaded while this block
     // is handled (its hash inserts and counter updates wait on memory).
     uint32_t nslot = 0u, nc = 0u;
@@ -355,7 +368,7 @@ __global__ __launch_bounds__(1024) void k_blocks(StepArgs a) {
         // wave serialises at one L2 channel).
         const uint64_t imask = __ballot(ins);
         if (lane == 0) {
-            tot[wid] = total;
+            tot[wid] = pad ? (total + 31u) & ~31u : total;
             ext[wid] = (uint32_t)__popcll(imask);
         }
         __syncthreads();
@@ -1354,6 +1367,7 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
     // ~90 us of 0.45 ms on the 8M slab in such waits in the collider loop.
     __shared__ SceneDev ssc;
     const uint32_t nh = a.cnt->n_home;
+    const bool pad = PERMUTE && block_ranges_aligned(a, nh);
     // Threads 32 tg .. 32 tg + 31 stage neighbour blocks tg, tg + T/32, ...
     // (T threads per workgroup).
     constexpr int kGroups = kG2PThreads / 32;
@@ -1559,6 +1573,10 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
                 at(a.nxt.pid, j) = at(a.cur.pid, i);
                 at(a.nxt.sid, j) = sid;
             }
+        }
+        if (PERMUTE && pad) {  // the slots up to the next 32-particle boundary stay holes
+            const uint32_t hole = jend + threadIdx.x;
+            if (hole < off + ((cnt + 31u) & ~31u)) at(a.nxt.sid, hole) = kInvalid;
         }
     }
 }

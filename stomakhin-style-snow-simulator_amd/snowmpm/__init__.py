@@ -66,7 +66,7 @@ class StepTimings(C.Structure):
 class Config(C.Structure):
     _fields_ = [("particle_capacity", C.c_size_t), ("hash_log2", C.c_int), ("track_fp", C.c_int),
                 ("device", C.c_int), ("permute_period", C.c_int), ("kappa", C.c_double),
-                ("p2g_chunk", C.c_int), ("g2p_threads", C.c_int)]
+                ("p2g_chunk", C.c_int), ("g2p_threads", C.c_int), ("dense_blocks", C.c_int)]
 
 
 class Request(C.Structure):
@@ -301,10 +301,10 @@ class Engine:
     """Device-resident MPM engine holding any number of scenes."""
 
     def __init__(self, capacity, track_fp=False, device=0, hash_log2=0, kappa=0.0, permute_period=0, p2g_chunk=0,
-                 g2p_threads=0):
+                 g2p_threads=0, dense_blocks=False):
         cfg = Config(particle_capacity=int(capacity), hash_log2=int(hash_log2), track_fp=int(bool(track_fp)),
                      device=int(device), permute_period=int(permute_period), kappa=float(kappa),
-                     p2g_chunk=int(p2g_chunk), g2p_threads=int(g2p_threads))
+                     p2g_chunk=int(p2g_chunk), g2p_threads=int(g2p_threads), dense_blocks=int(bool(dense_blocks)))
         h = C.c_void_p()
         _check(lib.mpm_create(C.byref(cfg), C.byref(h)))
         self.h = h
