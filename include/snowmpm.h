@@ -112,11 +112,12 @@ typedef struct {
     int p2g_chunk;            /* particles P2G stages per pass: 16, 32 or 64; 0 = 32 */
     int g2p_threads;          /* G2P workgroup size: 64, 128 or 256; 0 = 128.
                                  Results depend on neither; mpm_autotune chooses both */
-    int dense_blocks;         /* 0 (default): each home block's particles take a range that
-                                 starts on a 32-particle (128-byte) boundary whenever the holes
-                                 fit in the slots, capacity + capacity/4 per buffer (a re-sort
-                                 leaves the holes in the buffers); 1: ranges packed end to end,
-                                 capacity slots.  Results do not depend on it */
+    int dense_blocks;         /* 0 (default): the buffers hold capacity + capacity/4 slots and
+                                 each home block's particles take a range that starts on a
+                                 32-particle (128-byte) boundary whenever the holes (at most 31
+                                 per home block) fit in the extra quarter; a re-sort leaves them
+                                 in the buffers.  1: capacity slots, ranges packed end to end.
+                                 Results do not depend on it */
 } mpm_config;
 
 /* ---- engine lifetime ---- */

@@ -84,8 +84,9 @@ int Engine::init(const mpm_config& cfg) {
     cus_ = cus;
     const size_t N = cfg_.particle_capacity;
     // Home blocks take sorted ranges that start on 32-particle boundaries
-    // when the holes fit (k_blocks), so a re-sort leaves holes in the
-    // buffers: a quarter of the capacity is kept for them.
+    // when their holes fit in the slots beyond the capacity (k_blocks), so a
+    // re-sort leaves holes in the buffers: a quarter of the capacity is kept
+    // for them.
     slots_ = cfg_.dense_blocks ? N : std::min<size_t>((N + N / 4 + 64 + 63) & ~(size_t)63, ((size_t)1 << 30) - 64);
     if (slots_ < N) slots_ = N;
     launch_.grid_p = (int)std::min<size_t>((N + 255) / 256, (size_t)cus * 16);
@@ -199,7 +200,7 @@ StepArgs Engine::args(uint32_t pass_flag, int permute) const {
     a.kappa = (float)cfg_.kappa;
     a.permute = permute;
     a.project = any_project_ ? 1 : 0;
-    a.slots = (uint32_t)slots_;
+    a.hole_room = (uint32_t)(slots_ - cfg_.particle_capacity);
     a.align_blocks = cfg_.dense_blocks ? 0 : 1;
     if (implicit_active()) a.imp = imp_;
     return a;
@@ -328,9 +329,8 @@ int Engine::scene_add(const mpm_params& prm, const mpm_collider* coll, int ncoll
     // The new particles go after the buffer's last slot in use (holes and
     // removed scenes included until the next re-sort compacts them).
     if ((size_t)n0 + n > slots_) {
-        set_error("particle slots exceeded: " + std::to_string(n0) + " in use (removed scenes and block-aligned "
-                  "holes included until the next step) + " + std::to_string(n) + " > " + std::to_string(slots_) +
-                  "; step once to compact");
+        set_error("particle slots exceeded: " + std::to_string(n0) + " in use (removed scenes included until the "
+                  "next step compacts them) + " + std::to_string(n) + " > " + std::to_string(slots_));
         return MPM_ERR_CAPACITY;
     }
     // The caller's double arrays go to the GPU as they are; a kernel converts

@@ -285,14 +285,17 @@ __global__ __launch_bounds__(1024) void k_list(StepArgs a) {
 }
 
 // Sorted ranges of home blocks start on 32-particle (128-byte) boundaries
-// when the holes fit: at most 31 per home block beyond the particles, which
-// the buffer's slots in use (n_particles) bound.  Then every range G2P,
-// k_scatter and k_cellsort write covers whole lines of each field, not parts
-// of lines that a neighbouring range's workgroup also writes; after a re-sort
-// the buffers keep that layout, with holes (scene id kInvalid) between the
-// blocks.  The decision is per pass and the same in every kernel that asks.
+// when the holes fit: at most 31 per home block, within the slots the
+// buffers hold beyond the capacity (hole_room), so the binned particles
+// (at most the capacity) and their holes fit the slots, and the holes a
+// re-sort leaves never take room an added scene within the capacity needs.
+// Then every range G2P, k_scatter and k_cellsort write covers whole lines of
+// each field, not parts of lines that a neighbouring range's workgroup also
+// writes; after a re-sort the buffers keep that layout, with holes (scene id
+// kInvalid) between the blocks.  The decision is per pass and the same in
+// every kernel that asks.
 __device__ __forceinline__ bool block_ranges_aligned(const StepArgs& a, uint32_t nh) {
-    return a.align_blocks && (uint64_t)a.cnt->n_particles + 31ull * nh <= (uint64_t)a.slots;
+    return a.align_blocks && 31ull * nh <= (uint64_t)a.hole_room;
 }
 
 // Per home block (one wave): exclusive scan of the 64 cell counts, particle

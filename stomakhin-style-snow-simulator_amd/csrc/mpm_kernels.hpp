@@ -24,7 +24,7 @@ struct StepArgs {
     int permute;          // 1: G2P writes every particle to nxt in sorted order (and
                           //    compacts); 0: only x v F_E J_P [F_P] S, at index i
     int project;          // some scene has colliders with position projection
-    uint32_t slots;       // particle slots per buffer (= sorted positions)
+    uint32_t hole_room;   // particle slots per buffer beyond the capacity (room for holes)
     int align_blocks;     // home blocks' sorted ranges start on 32-particle boundaries when the holes fit
     ImpDev imp;           // semi-implicit solve (imp.gm null: explicit steps only)
 };
