@@ -170,6 +170,13 @@ int mpm_scene_sums(mpm_engine* e, int scene, double out7[7]);
  * [bin+sort, p2g, grid, g2p(+fp64 fix-up), end]; *nsteps = steps averaged. */
 int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps);
 
+/* Particle slots of the buffers (no counterpart in the reference; a
+ * diagnostic of the block-aligned layout, mpm_config.dense_blocks): *used =
+ * slots in use up to the last particle, the holes between home blocks and a
+ * removed scene's particles (until the next step compacts them) included;
+ * *total = slots per buffer. */
+int mpm_slots(mpm_engine* e, size_t* used, size_t* total);
+
 /* Issue-rate probe (no counterpart in the reference; a measurement for the
  * P2G bound): on `device`, waves_per_simd waves per SIMD each run
  * `iterations` iterations of 4 independent v_mfma_f32_16x16x4f32 (mode 0),

@@ -736,6 +736,10 @@ int mpm_kernel_times(mpm_engine* e, double ms[5], int* nsteps) {
     return MPM_OK;
 }
 
+int mpm_slots(mpm_engine* e, size_t* used, size_t* total) {
+    return (e && used && total) ? e->eng.slots(used, total) : MPM_ERR_ARG;
+}
+
 mpm_collider mpm_make_plane(const double point[3], const double normal[3], double friction) {
     mpm_collider c;
     std::memset(&c, 0, sizeof c);

@@ -100,6 +100,16 @@ class Engine {
         for (int j = 0; j < 5; ++j) ms[j] = kernel_ms_[j];
         if (nsteps) *nsteps = kernel_steps_;
     }
+    // Particle slots in use (holes and removed scenes' particles included
+    // until the next re-sort) and the slots of each buffer.
+    int slots(size_t* used, size_t* total) {
+        uint32_t n = 0;
+        const int rc = device_count(&n);
+        if (rc) return rc;
+        *used = n;
+        *total = slots_;
+        return 0;
+    }
 
   private:
     int dalloc(void** p, size_t bytes);

@@ -109,6 +109,7 @@ _SIGS = {
     "mpm_scene_status": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
     "mpm_scene_sums": (C.c_int, [C.c_void_p, C.c_int, _pd]),
     "mpm_kernel_times": (C.c_int, [C.c_void_p, _pd, C.POINTER(C.c_int)]),
+    "mpm_slots": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     "mpm_probe_issue": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, _pd]),
     "mpm_probe_stream": (C.c_int, [C.c_int, C.c_int, _pd]),
     "mpm_scene_steps": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_uint64)]),
@@ -500,6 +501,14 @@ class Engine:
         out = (C.c_double * 7)()
         _check(lib.mpm_scene_sums(self.h, int(sid), out))
         return np.array(out[:])
+
+    def slots(self):
+        """(slots in use, slots per buffer): in use counts up to the last
+        particle, holes of the block-aligned layout and a removed scene's
+        particles (until the next step) included (see mpm_slots)."""
+        used, total = C.c_size_t(), C.c_size_t()
+        _check(lib.mpm_slots(self.h, C.byref(used), C.byref(total)))
+        return used.value, total.value
 
     def kernel_times(self):
         ms = (C.c_double * 5)()

@@ -43,16 +43,24 @@ def _run(dense, scenes, late):
         keep = [sids[0]] + sids[2:] + [sl]
         out = [eng.download(s) for s in keep]
         st = [eng.status(s) for s in keep]
-    return out, st
+        eng.step(4)  # includes a re-sort (the period is 4): no removed particles left
+        used, total = eng.slots()
+        live = sum(o["x"].shape[0] for o in out)
+    return out, st, (used, total, live)
 
 
 def test_aligned_ranges_equal_dense_ranges():
     scenes = [_thin(snowmpm.generate_scene(3, seed=11, index=i, scale=0.12), 97) for i in range(4)]
     scenes.append(_thin(snowmpm.generate_scene(2, seed=11, scale=0.06), 101))
     late = _thin(snowmpm.generate_scene(0, seed=11, scale=0.3), 89)
-    a, sa = _run(False, scenes, late)
-    b, sb = _run(True, scenes, late)
+    a, sa, ua = _run(False, scenes, late)
+    b, sb, ub = _run(True, scenes, late)
     assert sa == sb and all(s == (0, 0) for s in sa)
+    # the aligned engine holds holes (its blocks' counts are not multiples of
+    # 32), within the quarter of the capacity kept for them; the dense one
+    # holds the particles end to end
+    assert ua[2] < ua[0] <= ua[1] and ua[0] - ua[2] <= ua[1] - ub[1]
+    assert ub[0] == ub[2] and ub[1] < ua[1]
     for da, db in zip(a, b):
         for k in ("x", "v", "Fe", "Jp", "V0"):
             assert np.array_equal(da[k], db[k]), k
