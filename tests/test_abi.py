@@ -83,6 +83,8 @@ def test_probe_rejects_bad_arguments_without_a_gpu():
     assert lib.mpm_probe_issue(0, 1, 3, 16, out) == snowmpm.ERR_ARG      # K not in 1, 2, 4, 8, 16
     assert lib.mpm_probe_issue(0, 1, 4, 0, out) == snowmpm.ERR_ARG       # no iterations
     assert lib.mpm_probe_issue(0, 1, 4, 16, None) == snowmpm.ERR_ARG
+    used, total = ctypes.c_size_t(), ctypes.c_size_t()
+    assert lib.mpm_slots(None, ctypes.byref(used), ctypes.byref(total)) == snowmpm.ERR_ARG  # no engine
     out3 = (ctypes.c_double * 3)()
     assert lib.mpm_probe_stream(0, -1, out3) == snowmpm.ERR_ARG          # no such mode
     assert lib.mpm_probe_stream(0, 3, out3) == snowmpm.ERR_ARG
