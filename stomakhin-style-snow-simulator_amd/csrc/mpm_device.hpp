@@ -132,15 +132,17 @@ struct GridDev {
 };
 
 struct BinDev {
-    uint32_t* pkey;        // [N] slot<<6 | cell, or kInvalid
-    uint32_t* prank;       // [N] rank within cell
-    uint32_t* sidx;        // [N] sorted position -> particle index
-    uint32_t* spid;        // [N] sorted position -> pid (sort key inside a cell)
+    // [S]: one entry per particle slot of a buffer (S = capacity + the room
+    // for the block-aligned layout's holes), by particle index or sorted position
+    uint32_t* pkey;        // [S] slot<<6 | cell, or kInvalid
+    uint32_t* prank;       // [S] rank within cell
+    uint32_t* sidx;        // [S] sorted position -> particle index
+    uint32_t* spid;        // [S] sorted position -> pid (sort key inside a cell)
 };
 
 struct FallbackDev {
-    uint32_t* pos;         // [N] sorted positions whose update is redone in FP64
-    float* L;              // [N*9] their velocity gradients
+    uint32_t* pos;         // [capacity] sorted positions whose update is redone in FP64
+    float* L;              // [capacity*9] their velocity gradients
 };
 
 // Semi-implicit solve state (allocated on the first step of a scene with
@@ -153,7 +155,7 @@ struct CRScene {            // per scene, in FP64
 struct ImpDev {
     float* gm;              // [H*64] node mass of the unknowns (0: massless or held by a collider)
     float* vec[kCRVecs];    // [H*3*64] each
-    float* G;               // [9][N] per-particle -V0 dP F^T for the force differential
+    float* G;               // [9][S] per-particle -V0 dP F^T for the force differential
     uint32_t* ord;          // [nblocks] slots of all blocks in key order
     uint64_t* skey;         // [nblocks] their keys
     double* dp;             // [nblocks*4] per-block dot-product partials (key order)
