@@ -171,6 +171,7 @@ class Engine {
     // are compacted away) every cfg_.permute_period steps, and on the first
     // step after scenes were added or removed.
     bool need_permute_ = true;
+    bool tiles_tuned_ = false;  // mpm_autotune chose the tile sizes
     uint64_t step_count_ = 0;
     hipGraphExec_t gexec_[8] = {};  // [(mov_ * 2 + con_) * 2 + permute]
     StepArgs gargs_[8]{};

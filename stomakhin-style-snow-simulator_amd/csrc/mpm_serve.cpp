@@ -175,6 +175,7 @@ int Engine::autotune(int spc, int chosen[4]) {
         cfg_.permute_period = best_period;
     }
     launch_ = best;
+    tiles_tuned_ = true;
     drop_graphs();
     if (chosen) {
         chosen[0] = best.grid_p;
@@ -198,6 +199,10 @@ static size_t bytes_per_particle(bool track) {
 }
 constexpr size_t kBytesPerBlock = 8 + 2 * 64 * 4 + 4 + 4 + 8 * 4 + 8 * 4 + kVelCh * 64 * 4 + 4 + 4;
 constexpr size_t kBytesPerHome = (size_t)kPartFloats * 4;
+// Resident particles at which serving switches to the engine's own tile
+// sizes, and back to the latency tiles (a gap, so the step graphs are not
+// re-captured at every chunk near the threshold).
+constexpr size_t kServeThroughputAbove = 4000000, kServeLatencyBelow = 3000000;
 
 Footprint Engine::pools() const {
     return Footprint{cfg_.particle_capacity * bytes_per_particle(cfg_.track_fp != 0), (size_t)(H_ / 2) * kBytesPerBlock,
@@ -299,6 +304,29 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
     auto add = [&](mpm_request& q, int* sid) {
         return scene_add(q.prm, q.colliders, q.ncolliders, q.n, q.x, q.v, q.m, nullptr, nullptr, nullptr, q.E, sid);
     };
+    // Tile sizes by load (results do not depend on them): while few
+    // particles are resident, each step is bound by one home block's
+    // latency, and 256-thread G2P workgroups and 64-particle P2G staging
+    // chunks shorten it (p50 / p99 at 10-50 requests/s about 12% lower); at
+    // saturation the engine's own tiles are faster.  Tiles the caller set or
+    // mpm_autotune chose are kept; the engine's are restored on return.
+    struct TileGuard {
+        LaunchCfg& l;
+        const int chunk, threads;
+        ~TileGuard() {
+            l.p2g_chunk = chunk;
+            l.g2p_threads = threads;
+        }
+    } tile_guard{launch_, launch_.p2g_chunk, launch_.g2p_threads};
+    const bool adapt_tiles = !cfg_.p2g_chunk && !cfg_.g2p_threads && !tiles_tuned_;
+    bool latency_tiles = true;
+    auto choose_tiles = [&](size_t resident) {
+        if (!adapt_tiles) return;
+        if (latency_tiles && resident > kServeThroughputAbove) latency_tiles = false;
+        if (!latency_tiles && resident < kServeLatencyBelow) latency_tiles = true;
+        launch_.p2g_chunk = latency_tiles ? 64 : tile_guard.chunk;
+        launch_.g2p_threads = latency_tiles ? 256 : tile_guard.threads;
+    };
     if (mode == 1) {
         for (int k = 0; k < nreq; ++k) {
             mpm_request& q = R[order[k]];
@@ -312,6 +340,7 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
             if ((rc = add(q, &sid))) return fail(rc);
             q.t_admit_s = now();
             max_bytes = std::max(max_bytes, (double)fp[order[k]].total());
+            choose_tiles(q.n);
             rc = step(q.steps, nullptr);
             if (rc == MPM_ERR_DEVICE) {
                 uint32_t bits = 0;
@@ -388,7 +417,12 @@ int Engine::serve(mpm_request* R, int nreq, int mode, int max_chunk, mpm_serve_s
             }
             max_res = std::max(max_res, (int)act.size());
             int K = max_chunk;
-            for (const auto& a : act) K = std::min(K, a.left);
+            size_t resident = 0;
+            for (const auto& a : act) {
+                K = std::min(K, a.left);
+                resident += R[a.req].n;
+            }
+            choose_tiles(resident);
             rc = step_graph(K);
             if (rc && rc != MPM_ERR_DEVICE) return fail(rc);
             const double tr = now();
