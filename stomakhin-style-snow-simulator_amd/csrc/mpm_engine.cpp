@@ -493,10 +493,13 @@ int Engine::step(int nsteps, mpm_timings* t) {
     if (rc) return rc;
     const bool timed = t != nullptr;
     if (timed) {
+        // Timing-only events: no system-scope fence (L2 write-back) at each
+        // record, which would add its own time between the stages measured.
+        // The stream is synchronized before the times are read.
         const size_t need = (size_t)nsteps * kStepEvents;
         while (events_.size() < need) {
             hipEvent_t ev;
-            HIP_TRY(hipEventCreate(&ev));
+            HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableSystemFence));
             events_.push_back(ev);
         }
     }
