@@ -531,41 +531,49 @@ __global__ __launch_bounds__(256) void k_cellsort(StepArgs a) {
 //   m_i   += w m,   p_i += w m v,
 //   f_i,c += S_c0 dwx wy wz + S_c1 wx dwy wz + S_c2 wx wy dwz   (S = -V0 P F^T)
 // are, for a group of 4 particles of the cell, sums of outer products: with
-// rows (a, b) = the 4x4 x/y nodes and columns (d, ch) = z plane x channel,
-//   C_M[(a,b),(d,ch)] += sum_k (wx_a wy_b)[k] (wz_d q_ch)[k],  q = (m, m v)
-//   C_F[(a,b),(d,c)]  += sum_k (dwx_a wy_b)(wz_d S_c0) + (wx_a dwy_b)(wz_d S_c1)
-//                        + (wx_a wy_b)(dwz_d S_c2)
-// i.e. four v_mfma_f32_16x16x4f32 per group (K = the 4 particles).  Column
+// rows (a, d) = the 4x4 x/z nodes and columns (b, ch) = y node x channel,
+//   C_M[(a,d),(b,ch)] += sum_k (wx_a wz_d)[k] (wy_b q_ch)[k],  q = (m, m v)
+//   C_F[(a,d),(b,c)]  += sum_k (dwx_a wz_d)(wy_b S_c0) + (wx_a wz_d)(dwy_b S_c1)
+//                        + (wx_a dwz_d)(wy_b S_c2)
+// i.e. four v_mfma_f32_16x16x4f32 per group (K = the 4 particles).  Channel
 // c = 3 of C_F is unused.  Lane l supplies A[l%16][l/16] and B[l/16][l%16]
-// and holds D[4(l/16)+r][l%16], r = 0..3.  Rows split as (a, b) = (i&3, i>>2)
-// and columns as (d, ch) = (j&3, j>>2), so lane l serves particle k = l/16 of
-// the group with x node a = z plane d = l&3 and y node b = channel
-// ch = (l>>2)&3, and holds the x nodes r = 0..3 of node (., k, l&3),
+// and holds D[4(l/16)+r][l%16], r = 0..3.  Rows split as (a, d) = (i&3, i>>2)
+// and columns as (b, ch) = (j&3, j>>2), so lane l serves particle k = l/16 of
+// the group with x node a = y node b = l&3 and z node d = channel
+// ch = (l>>2)&3, and holds the x nodes r = 0..3 of node (., l&3, l/16),
 // channel (l>>2)&3.
 //
 // A wave stages its block's particles (sorted: cell, then particle id) CHUNK
-// (16 or 32) at a time into its own LDS records (per-axis B-spline weights and
+// (16, 32 or 64) at a time into LDS records (per-axis B-spline weights and
 // derivatives of grid.cpp:24-58 in grid units, m, m v, S/dx): a lane fetches
-// its x/z terms with one b128; the six y/channel terms are the same for the 4
+// its x/y terms with one b128; the six z/channel terms are the same for the 4
 // lanes of a quad, which fetch them with one b64 each and share them by DPP
 // quad broadcasts.  A group past the end of a cell reads an all-zero record.
 // The fields of the next chunk are loaded into registers while the current
 // one is accumulated.  The cells of an x row (cx = 0..3 at fixed cy, cz) are
 // consecutive in sorted order; their tiles, shifted by cx, are summed in
-// registers (x nodes 0..7) and added to the wave's LDS region once per x row
-// by read-add-write.  A wave's LDS operations complete in order, so every
-// float sum has a fixed order.
-// The region holds only 4 node planes Z at a time (a ring indexed by Z mod 4):
-// cells come in z-major order, and a row of cells at cz adds to planes
-// cz..cz+3, so once rows reach cz every plane below cz is complete; it is
-// then written out as the block's partial sums for that plane and its ring
-// slot cleared for plane Z + 4.  All 7 planes of every home block are written.
+// registers (x nodes 0..6) and added to the node region once per x row.
+//
+// The node region is held in registers, 4 node planes Z at a time: lane
+// (g, n, ch) = (l/16, l&3, (l>>2)&3) holds plane Z = zb + g (zb: the lowest
+// plane not yet written), y nodes Y = n and n + 4 and channel ch of the mass
+// sums and of the force sums, x nodes 0..6 (28 floats).  Cells come in
+// z-major order and a row of cells at cz adds to planes cz..cz+3, the planes
+// of its own lane groups once zb = cz; its y nodes cy + b reach the lanes
+// that hold them by a rotation of each quad by cy (DPP).  When rows reach cz
+// every plane below is complete: plane zb is written out as the block's
+// partial sums for that plane (through a 384-float LDS plane buffer whose
+// padding stays 0, so every 128-byte line is written whole), its registers
+// are cleared and the region moves down one lane group.  All 7 planes of
+// every home block are written.  With no ring in LDS a wave needs only its
+// staged records and the plane buffer, so occupancy is bound by registers.
 // Summation order: the particles of a cell form groups of 4 counted from the
 // cell's first particle, and staging chunks end only at cell boundaries (or,
 // inside a cell longer than the chunk, after a whole number of groups), so no
 // group spans two chunks; the MFMAs of cells at even and at odd cx accumulate
 // into two separate register sets, added in a fixed order when the row is
-// deposited.  Every float sum is therefore the same for any chunk size.
+// deposited; rows are added to the region in sorted order.  Every float sum
+// is therefore the same for any chunk size.
 // A home block H's 7x7x7 node region splits into the pieces that fall into
 // the blocks H + t, t = (tx, ty, tz) in {0,1}^3: 4 nodes along an axis with
 // bit 0, 3 with bit 1.  part stores a home block's sums plane by plane
@@ -596,17 +604,10 @@ __host__ __device__ constexpr int plane_piece(int txy) {
 }
 static_assert(plane_piece(4) == kPlaneFloats, "padded pieces fill a plane");
 
-// Region of a wave: [x half 0..1][channel 0..6][Y 0..6][Z mod 4][x 0..3] (x
-// half h holds x nodes 4h..4h+3).  The channel stride, 112 floats, is 16
-// banks off a multiple of 32, so the b128 accesses of 8 consecutive lanes
-// (4 z planes x 2 channels) fall into distinct banks.
-constexpr int kP2GStrZ = 4, kP2GStrY = 4 * kP2GStrZ, kP2GStrCh = 7 * kP2GStrY, kP2GStrH = kGridCh * kP2GStrCh;
-constexpr int kP2GReg = 2 * kP2GStrH;      // floats per wave region
-static_assert(kP2GStrCh % 32 == 16, "channel stride: 8 lanes on distinct banks");
 constexpr int kP2GRec = 40;                // floats per staged particle record
 // record layout (floats):
-//   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wz_a, dwz_a
-//   [16+6c .. 16+6c+5]   c = 0..3: wy_c, dwy_c, q_c, S_c0, S_c1, S_c2
+//   [4a .. 4a+3]         a = 0..3: wx_a, dwx_a, wy_a, dwy_a
+//   [16+6c .. 16+6c+5]   c = 0..3: wz_c, dwz_c, q_c, S_c0, S_c1, S_c2
 //                        (q = m, m v0, m v1, m v2; S_3* = 0)
 // Lane (quad member) n reads the pair [16+6c+2n, +1]; member 3's pair is
 // never used (it overlaps the next record, or the 2 floats of slack).
@@ -614,34 +615,15 @@ constexpr int kP2GRecC = 16;
 // staging floats of a CHUNK: the records, the all-zero record, 2 floats of slack
 template <int CHUNK>
 constexpr int p2g_pst() { return (CHUNK + 1) * kP2GRec + 2; }
-constexpr int kP2GFlushQ = kPlaneFloats / 64;  // plane outputs per lane
-static_assert(kPlaneFloats % 64 == 0, "whole plane outputs per lane");
-
-// Plane table: element e of a plane's partial sums (piece txy at
-// plane_piece(txy), then [c][ly][lx]) is region index kPlaneSrc.e[e] +
-// (Z mod 4) kP2GStrZ.  The padding after each piece reads an x = 7 entry of
-// the region, which the deposits never change from 0, so every 128-byte
-// line of a plane is written whole.
-struct PlaneSrc {
-    uint16_t e[kPlaneFloats];
-};
-constexpr PlaneSrc make_plane_src() {
-    PlaneSrc t{};
-    for (int e = 0; e < kPlaneFloats; ++e) t.e[e] = (uint16_t)(kP2GStrH + 3);  // x node 7: always 0
-    for (int txy = 0; txy < 4; ++txy) {
-        const int nx = piece_n(txy, 0), ny = piece_n(txy, 1);
-        for (int c = 0; c < kGridCh; ++c)
-            for (int ly = 0; ly < ny; ++ly)
-                for (int lx = 0; lx < nx; ++lx) {
-                    const int X = 4 * (txy & 1) + lx, Y = 4 * (txy >> 1) + ly;
-                    t.e[plane_piece(txy) + c * nx * ny + ly * nx + lx] =
-                        (uint16_t)((X >> 2) * kP2GStrH + c * kP2GStrCh + Y * kP2GStrY + (X & 3));
-                }
-    }
-    return t;
+constexpr int kP2GFlushQ = kPlaneFloats / 128;  // float pairs of a plane per lane
+static_assert(kPlaneFloats % 128 == 0, "whole float pairs of a plane per lane");
+// Offsets in a plane of partial sums of the x/y node pieces (tx, ty) =
+// (0,0), (1,0), (0,1), (1,1) for channel c, y node ly of the piece:
+// 4 x nodes (tx = 0) from a 16-byte boundary, 3 (tx = 1) from a 4-byte one.
+__device__ __forceinline__ int plane_off(int txy, int c, int ly) {
+    const int nx = (txy & 1) ? 3 : 4, ny = (txy & 2) ? 3 : 4;
+    return plane_piece(txy) + c * nx * ny + ly * nx;
 }
-static_assert(kP2GReg < 65536, "16-bit plane table entries");
-__constant__ PlaneSrc kPlaneSrc = make_plane_src();
 
 typedef float p2g_v4f __attribute__((ext_vector_type(4)));
 
@@ -711,30 +693,57 @@ __device__ __forceinline__ void p2g_stage(const P2GRaw& r, float inv_dx, float* 
     bspline(g2 - floorf(g2), wz, dwz);
     float4* r4 = reinterpret_cast<float4*>(rec);
 #pragma unroll
-    for (int e = 0; e < 4; ++e) r4[e] = make_float4(wx[e], dwx[e], wz[e], dwz[e]);
+    for (int e = 0; e < 4; ++e) r4[e] = make_float4(wx[e], dwx[e], wy[e], dwy[e]);
     float2* r2 = reinterpret_cast<float2*>(rec + kP2GRecC);
-    r2[0] = make_float2(wy[0], dwy[0]);
+    r2[0] = make_float2(wz[0], dwz[0]);
     r2[1] = make_float2(m, S[0][0]);
     r2[2] = make_float2(S[0][1], S[0][2]);
-    r2[3] = make_float2(wy[1], dwy[1]);
+    r2[3] = make_float2(wz[1], dwz[1]);
     r2[4] = make_float2(q1, S[1][0]);
     r2[5] = make_float2(S[1][1], S[1][2]);
-    r2[6] = make_float2(wy[2], dwy[2]);
+    r2[6] = make_float2(wz[2], dwz[2]);
     r2[7] = make_float2(q2, S[2][0]);
     r2[8] = make_float2(S[2][1], S[2][2]);
-    r2[9] = make_float2(wy[3], dwy[3]);
+    r2[9] = make_float2(wz[3], dwz[3]);
     r2[10] = make_float2(q3, 0.0f);
     r2[11] = make_float2(0.0f, 0.0f);
 }
 
-// Adds lane values t[0..3] to the b128 at p (read-add-write).
-__device__ __forceinline__ void p2g_add4(float* p, float t0, float t1, float t2, float t3) {
-    float4 v = *reinterpret_cast<float4*>(p);
-    v.x += t0;
-    v.y += t1;
-    v.z += t2;
-    v.w += t3;
-    *reinterpret_cast<float4*>(p) = v;
+// Value of lane (n - R) & 3 of this lane's quad, n = this lane's place in it
+// (DPP quad_perm [-R, 1-R, 2-R, 3-R] mod 4).
+template <int R>
+__device__ __forceinline__ float quad_rot(float v) {
+    constexpr int ctl = ((4 - R) & 3) | (((5 - R) & 3) << 2) | (((6 - R) & 3) << 4) | (((7 - R) & 3) << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctl, 0xF, 0xF, true));
+}
+
+// Adds an x row of cells at cy = R (its sums sm, sf for y node R + n, x
+// nodes 0..6, in quad member n) to the region: quad member n holds y nodes
+// n (slot 0) and n + 4 (slot 1); y node R + ((n - R) & 3) is n for n >= R.
+// (Value selects, not branches: a store through a selected slot would keep
+// the region out of registers.)
+template <int R, bool MASS>
+__device__ __forceinline__ void p2g_row_add(float (&rm)[2][7], float (&rf)[2][7], const float (&sm)[7],
+                                            const float (&sf)[7], int n) {
+    const bool s0 = n >= R;
+#pragma unroll
+    for (int x = 0; x < 7; ++x) {
+        const float tm = R ? quad_rot<R>(sm[x]) : sm[x];
+        const float tf = R ? quad_rot<R>(sf[x]) : sf[x];
+        if (R == 0) {
+            if (MASS) rm[0][x] += tm;
+            rf[0][x] += tf;
+        } else {
+            if (MASS) {
+                const float a0 = rm[0][x] + tm, a1 = rm[1][x] + tm;
+                rm[0][x] = s0 ? a0 : rm[0][x];
+                rm[1][x] = s0 ? rm[1][x] : a1;
+            }
+            const float b0 = rf[0][x] + tf, b1 = rf[1][x] + tf;
+            rf[0][x] = s0 ? b0 : rf[0][x];
+            rf[1][x] = s0 ? rf[1][x] : b1;
+        }
+    }
 }
 
 // Length of the staging chunk that starts at sorted position c of a block of
@@ -752,32 +761,30 @@ __device__ __forceinline__ int p2g_chunk_len(int cbeg, int nb, int c) {
     return b > c ? b - c : CHUNK;
 }
 
+// At most 128 VGPRs, 4 waves per SIMD (the region in registers would
+// otherwise take ~134 and allow only 3; the few spills are outside the
+// per-cell loop).
 template <int CHUNK, bool FORCE>
-__global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_p2g(StepArgs a) {
     static_assert(CHUNK <= 64 && CHUNK % 4 == 0, "one staged particle per lane, whole groups");
     constexpr int kPst = p2g_pst<CHUNK>();
-    __shared__ __align__(16) float reg[kP2GReg];
+    __shared__ __align__(16) float plane[kPlaneFloats];
     __shared__ __align__(16) float pst[kPst];
     const int lane = (int)threadIdx.x;
     const int kq = lane >> 4, n = lane & 15, na = n & 3, nb = n >> 2;
-    // this lane's deposit address for x row (cy, cz) = (0, 0) without the
-    // ring slot: nodes (X, Y = cy + kq, Z = cz + na), channel nb (C_M) or
-    // 4 + nb (C_F)
-    const int dep0 = nb * kP2GStrCh + kq * kP2GStrY;
     const uint32_t nh = a.cnt->n_home;
     const uint32_t G = gridDim.x;
     uint32_t blk = blockIdx.x;
     if (blk >= nh) return;
-    for (int e = lane; e < kP2GReg / 4; e += 64) reinterpret_cast<float4*>(reg)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // the plane buffer's padding stays 0; the zero record
+    for (int e = lane; e < kPlaneFloats / 4; e += 64) reinterpret_cast<float4*>(plane)[e] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < kP2GRec + 2) pst[CHUNK * kP2GRec + lane] = 0.0f;
-    // this lane's plane outputs e = lane + 64 q: their region indices
-    // (ring slot 0), two per register
-    uint32_t fsrc[(kP2GFlushQ + 1) / 2];
+    // the node region: [y slot][x] of this lane's plane, y nodes and channel
+    float rm[2][7], rf[2][7];
 #pragma unroll
-    for (int q = 0; q < kP2GFlushQ; q += 2) {
-        const int e0 = lane + 64 * q, e1 = e0 + 64;
-        fsrc[q / 2] = kPlaneSrc.e[e0] | ((uint32_t)kPlaneSrc.e[e1] << 16);
-    }
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int x = 0; x < 7; ++x) rm[t][x] = rf[t][x] = 0.0f;
 
     // Chunk pipeline over this wave's blocks blk, blk + G, ...: while chunk k
     // is accumulated, the fields of chunk k + 1 and the sorted indices of
@@ -830,7 +837,7 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
     const int zero_b = (CHUNK - kq) * kP2GRec * 4;  // bytes from record kq to the zero record
     auto fetch = [&](int p, int hi, float4& x, float2& c) {
         const int b = kq < hi - p ? p * (kP2GRec * 4) : zero_b;  // hi - p and p * 160 are wave-uniform
-        x = *reinterpret_cast<const float4*>(lane_x + b);  // wx_a dwx_a wz_d dwz_d
+        x = *reinterpret_cast<const float4*>(lane_x + b);  // wx_a dwx_a wy_b dwy_b
         c = *reinterpret_cast<const float2*>(lane_c + b);
     };
 
@@ -842,21 +849,57 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         const bool write = blk < a.g.max_home;
         if (!write && lane == 0) flag(a.st, sid, kStatusOverflow);
         float* out = a.g.part + (size_t)(write ? blk : 0u) * kPartFloats;
-        // Node plane Z is complete: write it as the block's partial sums for
-        // that plane (384 consecutive floats, padding included) and clear
-        // its ring slot.
-        int zf = 0;  // next plane to write
-        auto flush_plane = [&](int Z) {
-            const int slot = (Z & 3) * kP2GStrZ;
-            float* po = out + Z * kPlaneFloats;
-#pragma unroll
-            for (int q = 0; q < kP2GFlushQ; ++q) {
-                const int e = lane + 64 * q;
-                float* src = reg + (int)((fsrc[q / 2] >> (16 * (q & 1))) & 0xFFFFu) + slot;
-                const float v = *src;
-                *src = 0.0f;
-                if (write) po[e] = v;
+        // Writes the plane lane group g holds as plane Z of the block's
+        // partial sums: the group's lanes put it into the plane buffer, then
+        // every lane stores 3 float pairs of it (whole 128-byte lines).
+        auto write_plane = [&](int g, int Z) __attribute__((always_inline)) {
+            if (kq == g) {
+                // plane buffer offsets of this lane's nodes: y node na (ty = 0)
+                // and na + 4 (ty = 1), x nodes 0..3 (tx = 0) and 4..6 (tx = 1);
+                // mass channel nb, force channel 4 + nb
+                const int pm00 = plane_off(0, nb, na), pm10 = plane_off(1, nb, na);
+                const int pm01 = plane_off(2, nb, na), pm11 = plane_off(3, nb, na);
+                const int pf00 = plane_off(0, 4 + nb, na), pf10 = plane_off(1, 4 + nb, na);
+                const int pf01 = plane_off(2, 4 + nb, na), pf11 = plane_off(3, 4 + nb, na);
+                *reinterpret_cast<float4*>(plane + pm00) = make_float4(rm[0][0], rm[0][1], rm[0][2], rm[0][3]);
+                plane[pm10] = rm[0][4], plane[pm10 + 1] = rm[0][5], plane[pm10 + 2] = rm[0][6];
+                if (na < 3) {
+                    *reinterpret_cast<float4*>(plane + pm01) = make_float4(rm[1][0], rm[1][1], rm[1][2], rm[1][3]);
+                    plane[pm11] = rm[1][4], plane[pm11 + 1] = rm[1][5], plane[pm11 + 2] = rm[1][6];
+                }
+                if (nb < 3) {
+                    *reinterpret_cast<float4*>(plane + pf00) = make_float4(rf[0][0], rf[0][1], rf[0][2], rf[0][3]);
+                    plane[pf10] = rf[0][4], plane[pf10 + 1] = rf[0][5], plane[pf10 + 2] = rf[0][6];
+                    if (na < 3) {
+                        *reinterpret_cast<float4*>(plane + pf01) =
+                            make_float4(rf[1][0], rf[1][1], rf[1][2], rf[1][3]);
+                        plane[pf11] = rf[1][4], plane[pf11 + 1] = rf[1][5], plane[pf11 + 2] = rf[1][6];
+                    }
+                }
             }
+            float2 v[kP2GFlushQ];  // (a wave's LDS operations complete in order)
+#pragma unroll
+            for (int q = 0; q < kP2GFlushQ; ++q) v[q] = reinterpret_cast<const float2*>(plane)[lane + 64 * q];
+            float2* po = reinterpret_cast<float2*>(out + Z * kPlaneFloats);
+            if (write)
+#pragma unroll
+                for (int q = 0; q < kP2GFlushQ; ++q) po[lane + 64 * q] = v[q];
+        };
+        // Plane zb is complete: write it, clear it and move the region down
+        // one lane group (group 3 then holds the empty plane zb + 4).
+        int zb = 0;
+        auto advance = [&]() __attribute__((always_inline)) {
+            write_plane(0, zb);
+            const int rot_src = ((lane + 16) & 63) << 2;  // ds_bpermute address: lane group g + 1
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int x = 0; x < 7; ++x) {
+                    const float m = kq == 0 ? 0.0f : rm[t][x], f = kq == 0 ? 0.0f : rf[t][x];
+                    rm[t][x] = __int_as_float(__builtin_amdgcn_ds_bpermute(rot_src, __float_as_int(m)));
+                    rf[t][x] = __int_as_float(__builtin_amdgcn_ds_bpermute(rot_src, __float_as_int(f)));
+                }
+            ++zb;
         };
         // x-row accumulators, C_M and C_F: cells at even cx add to x nodes
         // 0..5 (aE), cells at odd cx to x nodes 1..6 (aO, index x - 1)
@@ -865,18 +908,24 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
         for (int e = 0; e < 6; ++e) amE[e] = afE[e] = amO[e] = afO[e] = 0.0f;
         int row = 0;  // current x row (cy + 4 cz)
         int l = 0;    // next cell to visit
-        auto deposit_row = [&]() {
+        auto deposit_row = [&]() __attribute__((always_inline)) {
             const int cy = row & 3, cz = row >> 2;
-            for (; zf < cz; ++zf) flush_plane(zf);  // rows have reached cz: planes below it are complete
-            __asm__ __volatile__("" ::: "memory");
-            float* d = reg + dep0 + cy * kP2GStrY + ((cz + na) & 3) * kP2GStrZ;
-            p2g_add4(d, amE[0], amE[1] + amO[0], amE[2] + amO[1], amE[3] + amO[2]);
-            p2g_add4(d + kP2GStrH, amE[4] + amO[3], amE[5] + amO[4], amO[5], 0.0f);
-            if (nb < 3) {
-                p2g_add4(d + 4 * kP2GStrCh, afE[0], afE[1] + afO[0], afE[2] + afO[1], afE[3] + afO[2]);
-                p2g_add4(d + 4 * kP2GStrCh + kP2GStrH, afE[4] + afO[3], afE[5] + afO[4], afO[5], 0.0f);
-            }
-            __asm__ __volatile__("" ::: "memory");  // keep the rows' read-add-writes in program order
+            // rows have reached cz: planes below it are complete (cz - zb <= 3)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (zb < cz) advance();
+            const float sm[7] = {amE[0],         amE[1] + amO[0], amE[2] + amO[1], amE[3] + amO[2],
+                                 amE[4] + amO[3], amE[5] + amO[4], amO[5]};
+            const float sf[7] = {afE[0],         afE[1] + afO[0], afE[2] + afO[1], afE[3] + afO[2],
+                                 afE[4] + afO[3], afE[5] + afO[4], afO[5]};
+            if (cy == 0)
+                p2g_row_add<0, !FORCE>(rm, rf, sm, sf, na);
+            else if (cy == 1)
+                p2g_row_add<1, !FORCE>(rm, rf, sm, sf, na);
+            else if (cy == 2)
+                p2g_row_add<2, !FORCE>(rm, rf, sm, sf, na);
+            else
+                p2g_row_add<3, !FORCE>(rm, rf, sm, sf, na);
 #pragma unroll
             for (int e = 0; e < 6; ++e) amE[e] = afE[e] = amO[e] = afO[e] = 0.0f;
         };
@@ -898,6 +947,10 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
 #pragma unroll
                 for (int cx = 0; cx < 4; ++cx) {
                     if (!more || (l & 3) != cx) continue;
+                    if (cx == 0 && (l >> 2) != row) {  // a new x row: the rows before it are complete
+                        deposit_row();
+                        row = l >> 2;
+                    }
                     const int cb = __builtin_amdgcn_readlane(h.cbeg, l) - c0;
                     const int ce = __builtin_amdgcn_readlane(h.cend, l) - c0;
                     if (cb >= cn) {  // starts in a later chunk
@@ -906,10 +959,6 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                     }
                     const int lo = max(cb, 0), hi = min(ce, cn);
                     if (lo < hi) {
-                        if ((l >> 2) != row) {
-                            deposit_row();
-                            row = l >> 2;
-                        }
                         float4 x0, x1;
                         float2 q0, q1;
                         fetch(lo, hi, x0, q0);
@@ -928,14 +977,14 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
                             cf = p2g_v4f{afE[o], afE[o + 1], afE[o + 2], afE[o + 3]};
                         }
                         auto group = [&](const float4& rx, const float2& rc) {
-                            const float wy = quad_bcast<0>(rc.x), dwy = quad_bcast<0>(rc.y);
+                            const float wz = quad_bcast<0>(rc.x), dwz = quad_bcast<0>(rc.y);
                             const float q = quad_bcast<1>(rc.x), s0 = quad_bcast<1>(rc.y);
                             const float s1 = quad_bcast<2>(rc.x), s2 = quad_bcast<2>(rc.y);
-                            const float wxy = rx.x * wy;  // wx_a wy_b
-                            if (!FORCE) cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.z * q, cm, 0, 0, 0);
-                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wy, rx.z * s0, cf, 0, 0, 0);
-                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwy, rx.z * s1, cf, 0, 0, 0);
-                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxy, rx.w * s2, cf, 0, 0, 0);
+                            const float wxz = rx.x * wz;  // wx_a wz_d
+                            if (!FORCE) cm = __builtin_amdgcn_mfma_f32_16x16x4f32(wxz, rx.z * q, cm, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.y * wz, rx.z * s0, cf, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(wxz, rx.w * s1, cf, 0, 0, 0);
+                            cf = __builtin_amdgcn_mfma_f32_16x16x4f32(rx.x * dwz, rx.z * s2, cf, 0, 0, 0);
                         };
                         // the first two groups (a cell piece of <= 4 particles needs
                         // only the first: the second would read the zero record);
@@ -967,8 +1016,16 @@ __global__ __launch_bounds__(64) void k_p2g(StepArgs a) {
             if (last) break;
         }
         deposit_row();
-        for (; zf < 7; ++zf) flush_plane(zf);  // the planes of the last rows
-        __asm__ __volatile__("" ::: "memory");
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (zb < 3) advance();
+        // the planes of the last rows: zb..zb+3 = 3..6 in lane groups 0..3
+#pragma unroll
+        for (int g = 0; g < 4; ++g) write_plane(g, 3 + g);
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int x = 0; x < 7; ++x) rm[t][x] = rf[t][x] = 0.0f;
         // shift the headers and the chunk positions to the next block
         h = hn;
         hn = hnn;
