@@ -856,11 +856,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (kq == g) {
                 // plane buffer offsets of this lane's nodes: y node na (ty = 0)
                 // and na + 4 (ty = 1), x nodes 0..3 (tx = 0) and 4..6 (tx = 1);
-                // mass channel nb, force channel 4 + nb
-                const int pm00 = plane_off(0, nb, na), pm10 = plane_off(1, nb, na);
-                const int pm01 = plane_off(2, nb, na), pm11 = plane_off(3, nb, na);
-                const int pf00 = plane_off(0, 4 + nb, na), pf10 = plane_off(1, 4 + nb, na);
-                const int pf01 = plane_off(2, 4 + nb, na), pf11 = plane_off(3, 4 + nb, na);
+                // mass channel nb, force channel 4 + nb.  (Formed here from a
+                // lane id the compiler cannot see through, so they are not
+                // held in registers across the block.)
+                int ln = lane;
+                __asm__ __volatile__("" : "+v"(ln));
+                const int la = ln & 3, lb = (ln >> 2) & 3;
+                const int pm00 = plane_off(0, lb, la), pm10 = plane_off(1, lb, la);
+                const int pm01 = plane_off(2, lb, la), pm11 = plane_off(3, lb, la);
+                const int pf00 = plane_off(0, 4 + lb, la), pf10 = plane_off(1, 4 + lb, la);
+                const int pf01 = plane_off(2, 4 + lb, la), pf11 = plane_off(3, 4 + lb, la);
                 *reinterpret_cast<float4*>(plane + pm00) = make_float4(rm[0][0], rm[0][1], rm[0][2], rm[0][3]);
                 plane[pm10] = rm[0][4], plane[pm10 + 1] = rm[0][5], plane[pm10 + 2] = rm[0][6];
                 if (na < 3) {
