@@ -22,7 +22,9 @@ own CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-serving] [--no-seeding]
                        [--no-implicit]
-       python bench.py --batched-only --steps K   (the batched form alone, for profiling)
+       python bench.py --batched-only --steps K [--no-multi]
+                                                   (the batched form alone, for profiling; --no-multi:
+                                                   one engine only, without the two-engine leg)
        python bench.py --issue-probe               (MFMA/VALU issue rates behind the P2G bound)
        python bench.py --stream-probe              (HBM rate of a copy and of G2P's particle streams)
        python bench.py --config-baselines          (configs 0, 1, 3: plain GPU step beside the FP64 oracle)
@@ -378,7 +380,7 @@ def semi_implicit(seed, no_cpu, steps=20, warmup=2, beta=1.0, tol=1e-6, E=1e7, d
 
 
 def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(10, 25, 50, 100, 200),
-                       trace_s=2.5, min_requests=64, batched_only=False):
+                       trace_s=2.5, min_requests=64, batched_only=False, multi=True):
     """Batched form (256 uneven scenes in one engine, graph-replayed step,
     autotuned launch shapes) and served form (Poisson arrivals, continuous
     batching) against the plain per-scene GPU step one scene at a time.
@@ -405,6 +407,8 @@ def batched_and_served(seed, n_scenes=256, steps=1000, plain_sample=16, rates=(1
     out["batched_scenes_per_s"] = n_scenes / wall
     out["batched_ms_per_step"] = wall * 1e3 / steps
     out["batched_particle_steps_per_s"] = total * steps / wall
+    if not multi:
+        return out
     # the same scenes over two engines (alternating by size), stepped
     # together on two streams (mpm_step_graph_multi)
     halves = [np.argsort(ns)[k::2] for k in range(2)]
@@ -560,6 +564,8 @@ def main():
     ap.add_argument("--no-implicit", action="store_true", help="skip the semi-implicit measurement")
     ap.add_argument("--batched-only", action="store_true",
                     help="only the batched form (256 scenes), --steps graph-replayed steps each (for profiling)")
+    ap.add_argument("--no-multi", action="store_true",
+                    help="with --batched-only: one engine only, without the two-engine leg")
     ap.add_argument("--traffic-from", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="write profiles/traffic.json from two rocprofv3 counter CSVs (no GPU work)")
     ap.add_argument("--issue-probe", action="store_true",
@@ -593,7 +599,8 @@ def main():
         print(json.dumps({"configs_plain": config_baselines(args.seed, args.no_cpu)}))
         return
     if args.batched_only:
-        print(json.dumps({"batched": batched_and_served(args.seed, steps=args.steps, batched_only=True)}))
+        print(json.dumps({"batched": batched_and_served(args.seed, steps=args.steps, batched_only=True,
+                                                        multi=not args.no_multi)}))
         return
     t0 = time.perf_counter()
     sc = snowmpm.generate_scene(2, seed=args.seed, scale=1.0)

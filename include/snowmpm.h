@@ -109,7 +109,7 @@ typedef struct {
                                  particles at their index); 1 re-sorts every step.  Fast-moving
                                  scenes run ~3% faster at 4, scenes at rest ~3% slower */
     double kappa;             /* FP64 fallback when cond(trial F_E) > kappa; 0 = 1.5 */
-    int p2g_chunk;            /* particles P2G stages per pass: 16, 32 or 64; 0 = 32 */
+    int p2g_chunk;            /* particles P2G stages per pass: 16, 32, 48 or 64; 0 = 48 */
     int g2p_threads;          /* G2P workgroup size: 64, 128 or 256; 0 = 128.
                                  Results depend on neither; mpm_autotune chooses both */
     int dense_blocks;         /* 0 (default): the buffers hold capacity + capacity/4 slots and

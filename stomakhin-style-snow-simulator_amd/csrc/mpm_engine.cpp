@@ -70,9 +70,10 @@ int Engine::init(const mpm_config& cfg) {
         return MPM_ERR_ARG;
     }
     if (cfg_.permute_period == 0) cfg_.permute_period = 16;
-    if ((cfg_.p2g_chunk != 0 && cfg_.p2g_chunk != 16 && cfg_.p2g_chunk != 32 && cfg_.p2g_chunk != 64) ||
+    if ((cfg_.p2g_chunk != 0 && cfg_.p2g_chunk != 16 && cfg_.p2g_chunk != 32 && cfg_.p2g_chunk != 48 &&
+         cfg_.p2g_chunk != 64) ||
         (cfg_.g2p_threads != 0 && cfg_.g2p_threads != 64 && cfg_.g2p_threads != 128 && cfg_.g2p_threads != 256)) {
-        set_error("p2g_chunk must be 0, 16, 32 or 64 and g2p_threads 0, 64, 128 or 256");
+        set_error("p2g_chunk must be 0, 16, 32, 48 or 64 and g2p_threads 0, 64, 128 or 256");
         return MPM_ERR_ARG;
     }
     HIP_TRY(hipSetDevice(cfg_.device));
@@ -112,7 +113,7 @@ int Engine::init(const mpm_config& cfg) {
     // with >= 256 particles per occupied block (a dense 8-per-cell block has
     // 512); more home blocks than that fall back to the persistent loop.
     launch_.grid_h = (int)std::min<size_t>((N + 255) / 256, (size_t)1 << 20);
-    launch_.p2g_chunk = cfg_.p2g_chunk ? cfg_.p2g_chunk : 32;
+    launch_.p2g_chunk = cfg_.p2g_chunk ? cfg_.p2g_chunk : 48;
     launch_.g2p_threads = cfg_.g2p_threads ? cfg_.g2p_threads : 128;
     const bool track = cfg_.track_fp != 0;
 

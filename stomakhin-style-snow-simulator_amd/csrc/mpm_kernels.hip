@@ -1888,10 +1888,14 @@ inline int grid_grid(const LaunchCfg& c) { return c.grid_b > c.grid_h / 2 ? c.gr
 // G2P: a workgroup per home block as well.
 inline int g2p_grid(const LaunchCfg& c) { return c.grid_w > c.grid_h ? c.grid_w : c.grid_h; }
 
-// P2G with the staging chunk the launch configuration names (16, 32 or 64).
+// P2G with the staging chunk the launch configuration names (16, 32, 48 or
+// 64).  48 is the default: 48 of a wave's 64 lanes stage particles and the
+// records and plane buffer (9.2 KB of LDS) still let 16 waves share a CU.
 inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     if (c.p2g_chunk == 16)
         k_p2g<16, false><<<p2g_grid(c), 64, 0, s>>>(a);
+    else if (c.p2g_chunk == 48)
+        k_p2g<48, false><<<p2g_grid(c), 64, 0, s>>>(a);
     else if (c.p2g_chunk == 64)
         k_p2g<64, false><<<p2g_grid(c), 64, 0, s>>>(a);
     else
@@ -1900,6 +1904,8 @@ inline void launch_p2g(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
 void launch_p2g_force(const StepArgs& a, const LaunchCfg& c, hipStream_t s) {
     if (c.p2g_chunk == 16)
         k_p2g<16, true><<<p2g_grid(c), 64, 0, s>>>(a);
+    else if (c.p2g_chunk == 48)
+        k_p2g<48, true><<<p2g_grid(c), 64, 0, s>>>(a);
     else if (c.p2g_chunk == 64)
         k_p2g<64, true><<<p2g_grid(c), 64, 0, s>>>(a);
     else

@@ -36,7 +36,7 @@ struct LaunchCfg {
     int grid_l;  // hash-slot compaction (4096 slots per workgroup)
     int grid_k;  // k_blocks (16 blocks per workgroup)
     int grid_h;  // lower bound of the P2G grid (one wave per home block)
-    int p2g_chunk;    // particles P2G stages per pass: 16 or 32
+    int p2g_chunk;    // particles P2G stages per pass: 16, 32, 48 or 64
     int g2p_threads;  // G2P workgroup size: 64, 128 or 256
 };
 

@@ -112,7 +112,7 @@ int Engine::autotune(int spc, int chosen[4]) {
     // Tile sizes first (P2G staging chunk, G2P workgroup size), then the
     // persistent grid sizes, each by coordinate descent.
     {
-        const int chunks[] = {32, 16, 64}, threads[] = {128, 64, 256};
+        const int chunks[] = {48, 32, 64}, threads[] = {128, 64, 256};
         for (int k = 0; k < 3; ++k) {
             LaunchCfg c = best;
             c.p2g_chunk = chunks[k];

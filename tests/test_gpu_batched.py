@@ -155,7 +155,7 @@ def test_upload_download_round_trip():
 
 
 def test_tile_sizes_do_not_change_bits():
-    """P2G staging chunks of 16, 32 and 64 particles and G2P workgroups of 64,
+    """P2G staging chunks of 16, 32, 48 and 64 particles and G2P workgroups of 64,
     128 and 256 threads give bitwise the same state: on uniformly random
     positions (cells of 0 to ~35 particles, cells longer than a chunk) and on
     a scene of the batched distribution, through contact."""
@@ -167,7 +167,7 @@ def test_tile_sizes_do_not_change_bits():
     rand = dict(x=x, v=v, m=np.full(n, 4e-4), params=prm, colliders=[snowmpm.make_plane((0, 0, 0), (0, 1, 0), 0.2)])
     for sc in (rand, snowmpm.generate_scene(3, seed=5, index=7, scale=0.5)):
         out = []
-        for chunk, thr in ((32, 128), (16, 128), (64, 64), (32, 256)):
+        for chunk, thr in ((48, 128), (16, 128), (32, 128), (64, 64), (48, 256)):
             with snowmpm.Engine(capacity=len(sc["x"]) + 64, p2g_chunk=chunk, g2p_threads=thr) as eng:
                 sid = eng.add_scene(sc["params"], sc["colliders"], sc["x"], sc["v"], sc["m"])
                 assert eng.tile_sizes() == dict(p2g_chunk=chunk, g2p_threads=thr)
