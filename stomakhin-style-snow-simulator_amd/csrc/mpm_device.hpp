@@ -88,10 +88,13 @@ struct PBuf {
     uint32_t stride;  // elements from one component of x, v, F, Fp, S to the next
 };
 
-// The fields a step changes (x v F_E J_P F_P S) and the ones it does not
-// (m V0 E pid sid) of a particle view may come from different buffers.
+// The fields every step changes (x v F_E S) and the others (m V0 E pid sid,
+// and J_P F_P, which only the plastic update changes and an ordinary step
+// updates in place) of a particle view may come from different buffers.
 inline PBuf compose_pbuf(const PBuf& moving, const PBuf& constant) {
     PBuf p = moving;
+    p.Jp = constant.Jp;
+    for (int e = 0; e < 9; ++e) p.Fp[e] = constant.Fp[e];
     p.m = constant.m;
     p.V0 = constant.V0;
     p.E = constant.E;

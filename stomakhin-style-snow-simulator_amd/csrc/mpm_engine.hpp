@@ -147,9 +147,10 @@ class Engine {
     // (128-byte) boundary.
     size_t slots_ = 0;
     PBuf buf_[2]{};
-    // Current particle state: moving fields in buf_[mov_], constant fields in
-    // buf_[con_].  Every step writes the moving fields to the other buffer;
-    // a permuting step also moves the constant fields.
+    // Current particle state: moving fields (x v F_E S) in buf_[mov_], the
+    // others in buf_[con_].  Every step writes the moving fields to the other
+    // buffer and updates J_P and F_P in place where the plastic update changed
+    // them; a permuting step moves all fields to the other buffers.
     int mov_ = 0;
     int con_ = 0;
     GridDev grid_{};

@@ -1568,6 +1568,15 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
             Sym Es;
             const bool elastic = elastic_trial(Fh, sc.theta_c, sc.theta_s, Es);
             bool fb = false;
+            // Outputs go to sorted position j of the next buffers on a
+            // permuting pass, else to the particle's own index i.
+            const PBuf& o = a.nxt;
+            const uint32_t w = PERMUTE ? j : i;
+            // J_P and F_P change only in the plastic update (a deferred
+            // particle's are written by the FP64 fix-up): outside re-sort steps
+            // the output view shares their buffer with the input, so they are
+            // written only where the update changed them (stored here, in the
+            // branch that forms them, to keep their registers short-lived).
             if (elastic) {
                 const float h = expf(sc.hardening * (1.0f - Jp));
 #pragma unroll
@@ -1576,10 +1585,22 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
                     Fp_new[e] = Fp_old[e];
                 }
                 po.Jp = Jp;
+                if (PERMUTE) {
+                    at(o.Jp, w) = Jp;
+                    if (track)
+#pragma unroll
+                        for (int e = 0; e < 9; ++e) comp(o.Fp, o, e, w) = Fp_old[e];
+                }
                 elastic_stress(Es, det3<float>(Fh), mu0 * h, la0 * h, V0, po.S);
             } else {
                 fb = plastic_update<TRACK>(Fe, L, dt, Jp, Fp_old, Fp_new, true, sc.theta_c, sc.theta_s, mu0, la0,
                                            sc.hardening, V0, a.kappa, po, true);
+                if (PERMUTE || !fb) {
+                    at(o.Jp, w) = fb ? Jp : po.Jp;
+                    if (track)
+#pragma unroll
+                        for (int e = 0; e < 9; ++e) comp(o.Fp, o, e, w) = fb ? Fp_old[e] : Fp_new[e];
+                }
             }
             if (fb) {
                 // Redone in FP64 at the start of k_end; keep finite placeholders meanwhile.
@@ -1599,20 +1620,13 @@ __global__ __launch_bounds__(kG2PThreads) __attribute__((amdgpu_waves_per_eu(4, 
             }
 
             // Permuting pass: the particle moves to sorted position j of the
-            // next buffers with all its fields.  Otherwise only the fields a
-            // step changes are written, at the particle's own index i, into
-            // the other buffer of the moving fields (nxt never aliases cur).
-            // The deformation state is stored first, which frees its registers
+            // next buffers with all its fields.  Otherwise the moving fields
+            // are written at the particle's own index i into the other buffer
+            // of the moving fields (which never aliases the input).  The
+            // deformation state is stored first, which frees its registers
             // before the velocity update.
-            const PBuf& o = a.nxt;
-            const uint32_t w = PERMUTE ? j : i;
 #pragma unroll
             for (int e = 0; e < 9; ++e) comp(o.F, o, e, w) = po.Fe[e];
-            at(o.Jp, w) = po.Jp;
-            if (track) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e) comp(o.Fp, o, e, w) = Fp_new[e];
-            }
 #pragma unroll
             for (int e = 0; e < 6; ++e) comp(o.S, o, e, w) = po.S[e];
 
